@@ -1,0 +1,614 @@
+/*
+ * CPU oracle for the D3Q19 BGK hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, FP64 restatement of the reference's serial time step, used by tests/, by the smoke
+ * check and by bench.py's CPU-baseline leg as the checker.  It is never linked into, called by or
+ * shipped with the HIP library.  Compile with -ffp-contract=off: every expression below is written
+ * in the reference's evaluation order so that the result is bit-identical to the reference built
+ * with the same flag.
+ *
+ * State layout: one contiguous array of 19 populations, q-major; population q (1-based) occupies
+ * a[(q-1)*np .. q*np-1] with np = (l+2)(m+2)(n+2) and site (i,j,k), i in 0..l+1 etc., at offset
+ * i + (l+2)*(j + (m+2)*k) -- the reference's ghosted Fortran arrays a01..a19 laid end to end
+ * (CODE/SRC/alloca.F90:94-133).  obs is int32 (l,m,n), i fastest, 1 = solid.
+ *
+ * Functions and the reference code they restate:
+ *   orc_parity_steps   CODE/SRC/bgk3d.F90:114-138 (order), bcond.obs.F90:47-77, bcond.bc.F90:127-205
+ *                      (outflow + border fix), :260-321 (inflow), :385-409/:472-496 (z periodic),
+ *                      :535-560/:600-625 (y periodic incl. corners), movef.F90:43-68, col.F90:94-229
+ *   orc_flux_x/_y      CODE/SRC/flux_X.F90:50-301, flux_Y.F90:51-282 (SERIAL offsets = 0)
+ *   orc_probe          CODE/SRC/probe_global.F90:41-77
+ *   orc_diagno         CODE/SRC/diagno.F90:53-131
+ *   orc_physics_steps  the same collision and boundary arithmetic with y/z periodicity applied by
+ *                      index wrap (ghosts fresh before bounce-back), solid sites not collided.
+ *   orc_mem_force      momentum-exchange force on the solid (no reference counterpart).
+ *
+ * Parity pins (see tests/test_oracle_*.py): bit-equality with the reference executable built by
+ * oracle/Makefile.ref on seeded inputs; closed forms (equilibrium moments, rest / uniform-flow
+ * fixed points, index-coded streaming permutation, single-solid-site bounce-back).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* weights as the reference computes them (quad precision, then rounded once to double;
+ * CODE/SRC/storage.F90:55-82).  1/3, 1/18 and 1/36 are the correctly rounded doubles. */
+static const double P0 = 1.0 / 3.0;
+static const double P1 = 1.0 / 18.0;
+static const double P2 = 1.0 / 36.0;
+static const double RF = 3.0;
+static const double QF = 1.5;
+
+static const int CX[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+static const int CY[19] = {-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1, 0};
+static const int CZ[19] = {0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1, 0};
+static const int OPP[19] = {11, 12, 9, 10, 13, 14, 15, 16, 17, 2, 3, 0, 1, 4, 5, 6, 7, 8, 18};
+
+typedef struct {
+    int l, m, n;
+    long lp, mp, np; /* l+2, m+2, total sites */
+} geom_t;
+
+static geom_t mkgeom(int l, int m, int n)
+{
+    geom_t g;
+    g.l = l; g.m = m; g.n = n;
+    g.lp = l + 2; g.mp = m + 2;
+    g.np = (long)(l + 2) * (m + 2) * (n + 2);
+    return g;
+}
+
+#define IDX(g, i, j, k) ((long)(i) + (g).lp * ((long)(j) + (g).mp * (long)(k)))
+#define A(q, i, j, k) a[(long)((q) - 1) * g.np + IDX(g, i, j, k)]
+#define B(q, i, j, k) b[(long)((q) - 1) * g.np + IDX(g, i, j, k)]
+#define OBS(i, j, k) obs[((long)(i) - 1) + (long)g.l * (((long)(j) - 1) + (long)g.m * ((long)(k) - 1))]
+
+/* ------------------------------------------------------------------------------------------ */
+/* BGK collision of one site: x[19] pre-collision -> out[19]; reference col.F90:127-229       */
+/* ------------------------------------------------------------------------------------------ */
+static void collide_site(const double *x, double *out, double omega, double fgrad)
+{
+    const double omega1 = 1.0 - omega;
+    double rho = x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7]
+               + x[8] + x[9] + x[10] + x[11] + x[12] + x[13] + x[14] + x[15]
+               + x[16] + x[17] + x[18];
+    double rhoinv = 1.0 / rho;
+    double vx = (x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) * rhoinv;
+    double vy = (x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]) * rhoinv;
+    double vz = (x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]) * rhoinv;
+    double vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+    double vsq = vx2 + vy2 + vz2;
+    double vxpy = vx + vy;
+    double qxpy = 1.0 + QF * (3.0 * vxpy * vxpy - vsq);
+    double vxmy = vx - vy;
+    double qxmy = 1.0 + QF * (3.0 * vxmy * vxmy - vsq);
+    double vxpz = vx + vz;
+    double qxpz = 1.0 + QF * (3.0 * vxpz * vxpz - vsq);
+    double vxmz = vx - vz;
+    double qxmz = 1.0 + QF * (3.0 * vxmz * vxmz - vsq);
+    double vypz = vy + vz;
+    double qypz = 1.0 + QF * (3.0 * vypz * vypz - vsq);
+    double vymz = vy - vz;
+    double qymz = 1.0 + QF * (3.0 * vymz * vymz - vsq);
+    double qx = 1.0 + QF * (3.0 * vx2 - vsq);
+    double qy = 1.0 + QF * (3.0 * vy2 - vsq);
+    double qz = 1.0 + QF * (3.0 * vz2 - vsq);
+    vx = RF * vx; vy = RF * vy; vz = RF * vz;
+    vxpy = RF * vxpy; vxmy = RF * vxmy; vxpz = RF * vxpz;
+    vxmz = RF * vxmz; vypz = RF * vypz; vymz = RF * vymz;
+    double rp1 = rho * P1;
+    double rp2 = rho * P2;
+    /* FORCING_X (default): force along x only (col.F90:173-187) */
+    double fx = fgrad * rho;
+    const double fy = 0.0, fz = 0.0;
+    double e[19];
+    e[0] = rp2 * (vxmy + qxmy);
+    e[1] = rp2 * (vxmz + qxmz);
+    e[2] = rp2 * (vxpy + qxpy);
+    e[3] = rp2 * (vxpz + qxpz);
+    e[4] = rp1 * (vx + qx);
+    e[5] = rp1 * (vz + qz);
+    e[6] = rp2 * (vypz + qypz);
+    e[7] = rp1 * (vy + qy);
+    e[8] = rp2 * (vymz + qymz);
+    e[9] = rp2 * (-vxpy + qxpy);
+    e[10] = rp2 * (-vxpz + qxpz);
+    e[11] = rp2 * (-vxmy + qxmy);
+    e[12] = rp2 * (-vxmz + qxmz);
+    e[13] = rp1 * (-vx + qx);
+    e[14] = rp1 * (-vz + qz);
+    e[15] = rp2 * (-vypz + qypz);
+    e[16] = rp1 * (-vy + qy);
+    e[17] = rp2 * (-vymz + qymz);
+    e[18] = rho * P0 * (1.0 - QF * vsq);
+    out[0] = omega1 * x[0] + (omega * e[0] + fx - fy);
+    out[1] = omega1 * x[1] + (omega * e[1] + fx - fz);
+    out[2] = omega1 * x[2] + (omega * e[2] + fx + fy);
+    out[3] = omega1 * x[3] + (omega * e[3] + fx + fz);
+    out[4] = omega1 * x[4] + (omega * e[4] + fx);
+    out[5] = omega1 * x[5] + (omega * e[5] + fz);
+    out[6] = omega1 * x[6] + (omega * e[6] + fy + fz);
+    out[7] = omega1 * x[7] + (omega * e[7] + fy);
+    out[8] = omega1 * x[8] + (omega * e[8] + fy - fz);
+    out[9] = omega1 * x[9] + (omega * e[9] - fx - fy);
+    out[10] = omega1 * x[10] + (omega * e[10] - fx - fz);
+    out[11] = omega1 * x[11] + (omega * e[11] - fx + fy);
+    out[12] = omega1 * x[12] + (omega * e[12] - fx + fz);
+    out[13] = omega1 * x[13] + (omega * e[13] - fx);
+    out[14] = omega1 * x[14] + (omega * e[14] - fz);
+    out[15] = omega1 * x[15] + (omega * e[15] - fy - fz);
+    out[16] = omega1 * x[16] + (omega * e[16] - fy);
+    out[17] = omega1 * x[17] + (omega * e[17] - fy + fz);
+    out[18] = omega1 * x[18] + (omega * e[18]);
+}
+
+/* outlet ghost values for q = 10..14 from the (not rho-divided) momentum of an interior site;
+ * reference bcond.bc.F90:138-164 */
+static void outlet_eq(const double *x, double *o /* o[0..4] = q10..q14 */)
+{
+    double xj = +(x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]);
+    double yj = +(x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]);
+    double zj = +(x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]);
+    double cvsq = xj * xj + yj * yj + zj * zj;
+    double c10 = RF * (-xj - yj) + QF * (3.0 * (xj + yj) * (xj + yj) - cvsq);
+    double c11 = RF * (-xj - zj) + QF * (3.0 * (xj + zj) * (xj + zj) - cvsq);
+    double c12 = RF * (-xj + yj) + QF * (3.0 * (xj - yj) * (xj - yj) - cvsq);
+    double c13 = RF * (-xj + zj) + QF * (3.0 * (xj - zj) * (xj - zj) - cvsq);
+    double c14 = RF * (-xj) + QF * (3.0 * (xj) * (xj)-cvsq);
+    o[0] = P2 * (1.0 + c10);
+    o[1] = P2 * (1.0 + c11);
+    o[2] = P2 * (1.0 + c12);
+    o[3] = P2 * (1.0 + c13);
+    o[4] = P1 * (1.0 + c14);
+}
+
+/* inlet ghost values for q = 1..5 at u = (u_in, 0, 0); reference bcond.bc.F90:260-321 */
+static void inlet_eq(double u_in, double *o /* o[0..4] = q1..q5 */)
+{
+    double xj = u_in, yj = 0.0, zj = 0.0;
+    double cvsq = xj * xj + yj * yj + zj * zj;
+    double c1 = RF * (xj - yj) + QF * (3.0 * (xj - yj) * (xj - yj) - cvsq);
+    double c2 = RF * (xj - zj) + QF * (3.0 * (xj - zj) * (xj - zj) - cvsq);
+    double c3 = RF * (xj + yj) + QF * (3.0 * (xj + yj) * (xj + yj) - cvsq);
+    double c4 = RF * (xj + zj) + QF * (3.0 * (xj + zj) * (xj + zj) - cvsq);
+    double c5 = RF * (xj) + QF * (3.0 * (xj) * (xj)-cvsq);
+    o[0] = P2 * (1.0 + c1);
+    o[1] = P2 * (1.0 + c2);
+    o[2] = P2 * (1.0 + c3);
+    o[3] = P2 * (1.0 + c4);
+    o[4] = P1 * (1.0 + c5);
+}
+
+static void gather(const double *a, const geom_t g, long s, double *x)
+{
+    for (int q = 0; q < 19; ++q) x[q] = a[q * g.np + s];
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* parity mode: the serial reference step, bit for bit                                        */
+/* ------------------------------------------------------------------------------------------ */
+static void bcond_obs(double *a, const int32_t *obs, geom_t g, const int bb[6])
+{
+    for (int k = bb[4]; k <= bb[5]; ++k)
+        for (int j = bb[2]; j <= bb[3]; ++j)
+            for (int i = bb[0]; i <= bb[1]; ++i) {
+                if (OBS(i, j, k) != 1) continue;
+                /* sequential in-place order as bcond.obs.F90:56-73 */
+                for (int q = 0; q < 18; ++q) {
+                    int p = OPP[q];
+                    A(q + 1, i, j, k) = A(p + 1, i + CX[q], j + CY[q], k + CZ[q]);
+                }
+            }
+}
+
+static void bcond_bc(double *a, geom_t g, double u_in, int outflow, int inflow)
+{
+    const int l = g.l, m = g.m, n = g.n, l1 = l + 1, m1 = m + 1, n1 = n + 1;
+    double x[19], o[5];
+    if (outflow) {
+        for (int k = 1; k <= n; ++k)
+            for (int j = 1; j <= m; ++j) {
+                gather(a, g, IDX(g, l, j, k), x);
+                outlet_eq(x, o);
+                for (int t = 0; t < 5; ++t) A(10 + t, l1, j, k) = o[t];
+            }
+        /* border fix (bcond.bc.F90:171-204) */
+        for (int j = 1; j <= m; ++j)
+            for (int t = 0; t < 5; ++t) {
+                A(10 + t, l1, j, 0) = A(10 + t, l1, j, 1);
+                A(10 + t, l1, j, n1) = A(10 + t, l1, j, n);
+            }
+        for (int k = 1; k <= n; ++k)
+            for (int t = 0; t < 5; ++t) {
+                A(10 + t, l1, 0, k) = A(10 + t, l1, 1, k);
+                A(10 + t, l1, m1, k) = A(10 + t, l1, m, k);
+            }
+    }
+    if (inflow) {
+        inlet_eq(u_in, o);
+        for (int k = 0; k <= n1; ++k)
+            for (int j = 0; j <= m1; ++j)
+                for (int t = 0; t < 5; ++t) A(1 + t, 0, j, k) = o[t];
+    }
+    /* up (z = n+1) periodic, interior range */
+    for (int j = 1; j <= m; ++j)
+        for (int i = 1; i <= l; ++i) {
+            A(2, i, j, n1) = A(2, i, j, 1);
+            A(9, i, j, n1) = A(9, i, j, 1);
+            A(11, i, j, n1) = A(11, i, j, 1);
+            A(15, i, j, n1) = A(15, i, j, 1);
+            A(16, i, j, n1) = A(16, i, j, 1);
+        }
+    /* down (z = 0) periodic */
+    for (int j = 1; j <= m; ++j)
+        for (int i = 1; i <= l; ++i) {
+            A(4, i, j, 0) = A(4, i, j, n);
+            A(6, i, j, 0) = A(6, i, j, n);
+            A(7, i, j, 0) = A(7, i, j, n);
+            A(13, i, j, 0) = A(13, i, j, n);
+            A(18, i, j, 0) = A(18, i, j, n);
+        }
+    /* left (y = 0) periodic with the SERIAL corner copy */
+    for (int k = 1; k <= n; ++k) {
+        A(3, 0, 0, k) = A(3, l, m, k);
+        for (int i = 1; i <= l; ++i) {
+            A(3, i, 0, k) = A(3, i, m, k);
+            A(7, i, 0, k) = A(7, i, m, k);
+            A(8, i, 0, k) = A(8, i, m, k);
+            A(9, i, 0, k) = A(9, i, m, k);
+            A(12, i, 0, k) = A(12, i, m, k);
+        }
+    }
+    /* right (y = m+1) periodic with the SERIAL corner copy */
+    for (int k = 1; k <= n; ++k) {
+        A(10, l1, m1, k) = A(10, 1, 1, k);
+        for (int i = 1; i <= l; ++i) {
+            A(1, i, m1, k) = A(1, i, 1, k);
+            A(10, i, m1, k) = A(10, i, 1, k);
+            A(16, i, m1, k) = A(16, i, 1, k);
+            A(17, i, m1, k) = A(17, i, 1, k);
+            A(18, i, m1, k) = A(18, i, 1, k);
+        }
+    }
+}
+
+static void movef(const double *a, double *b, geom_t g)
+{
+    for (int k = 1; k <= g.n; ++k)
+        for (int j = 1; j <= g.m; ++j)
+            for (int i = 1; i <= g.l; ++i)
+                for (int q = 0; q < 18; ++q)
+                    B(q + 1, i, j, k) = A(q + 1, i - CX[q], j - CY[q], k - CZ[q]);
+}
+
+static void col(double *a, const double *b, geom_t g, double omega, double fgrad)
+{
+    double x[19], out[19];
+    for (int k = 1; k <= g.n; ++k)
+        for (int j = 1; j <= g.m; ++j)
+            for (int i = 1; i <= g.l; ++i) {
+                long s = IDX(g, i, j, k);
+                for (int q = 0; q < 18; ++q) x[q] = b[q * g.np + s];
+                x[18] = a[18 * g.np + s];
+                collide_site(x, out, omega, fgrad);
+                for (int q = 0; q < 19; ++q) a[q * g.np + s] = out[q];
+            }
+}
+
+static void compute_bbox(const int32_t *obs, geom_t g, int bb[6])
+{
+    /* initial values as build.obs.F90:73-79 */
+    bb[0] = g.l; bb[1] = 0; bb[2] = g.m; bb[3] = 0; bb[4] = g.n; bb[5] = 0;
+    for (int k = 1; k <= g.n; ++k)
+        for (int j = 1; j <= g.m; ++j)
+            for (int i = 1; i <= g.l; ++i)
+                if (OBS(i, j, k) == 1) {
+                    if (i < bb[0]) bb[0] = i;
+                    if (i > bb[1]) bb[1] = i;
+                    if (j < bb[2]) bb[2] = j;
+                    if (j > bb[3]) bb[3] = j;
+                    if (k < bb[4]) bb[4] = k;
+                    if (k > bb[5]) bb[5] = k;
+                }
+}
+
+/* a: state (ghosted, 19*np); b: scratch of the same size; returns 0 */
+int orc_parity_steps(double *a, double *b, const int32_t *obs, int l, int m, int n, double omega,
+                     double fgrad, double u_inflow, int nsteps)
+{
+    geom_t g = mkgeom(l, m, n);
+    int bb[6];
+    compute_bbox(obs, g, bb);
+    for (int t = 0; t < nsteps; ++t) {
+        bcond_obs(a, obs, g, bb);
+        bcond_bc(a, g, u_inflow, 1, 1);
+        movef(a, b, g);
+        col(a, b, g, omega, fgrad);
+    }
+    return 0;
+}
+
+/* individual phases, for operator-level tests */
+void orc_phase_bcond_obs(double *a, const int32_t *obs, int l, int m, int n)
+{
+    geom_t g = mkgeom(l, m, n);
+    int bb[6];
+    compute_bbox(obs, g, bb);
+    bcond_obs(a, obs, g, bb);
+}
+void orc_phase_bcond_bc(double *a, int l, int m, int n, double u_inflow)
+{
+    bcond_bc(a, mkgeom(l, m, n), u_inflow, 1, 1);
+}
+void orc_phase_movef(const double *a, double *b, int l, int m, int n)
+{
+    movef(a, b, mkgeom(l, m, n));
+}
+void orc_phase_col(double *a, const double *b, int l, int m, int n, double omega, double fgrad)
+{
+    col(a, b, mkgeom(l, m, n), omega, fgrad);
+}
+void orc_collide_site(const double *x, double *out, double omega, double fgrad)
+{
+    collide_site(x, out, omega, fgrad);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* physics mode: periodic y/z by index wrap, inlet/outlet as above, bounce-back from the site's */
+/* own opposite population, solid sites not updated                                            */
+/* ------------------------------------------------------------------------------------------ */
+static inline int wrap(int v, int n) { return v < 1 ? v + n : (v > n ? v - n : v); }
+
+/* a: state (interior used), b: scratch; inflow/outflow: 1 = inlet at x=0 / outlet at x=l+1,
+ * 0 = periodic in x. */
+int orc_physics_steps(double *a, double *b, const int32_t *obs, int l, int m, int n, double omega,
+                      double fgrad, double u_inflow, int x_open, int nsteps)
+{
+    geom_t g = mkgeom(l, m, n);
+    double inl[5], x[19], out[19], y[19], o[5];
+    inlet_eq(u_inflow, inl);
+    for (int t = 0; t < nsteps; ++t) {
+        for (int k = 1; k <= n; ++k)
+            for (int j = 1; j <= m; ++j)
+                for (int i = 1; i <= l; ++i) {
+                    long s = IDX(g, i, j, k);
+                    if (OBS(i, j, k) == 1) {
+                        for (int q = 0; q < 19; ++q) b[q * g.np + s] = a[q * g.np + s];
+                        continue;
+                    }
+                    for (int q = 0; q < 18; ++q) {
+                        int si = i - CX[q], sj = wrap(j - CY[q], m), sk = wrap(k - CZ[q], n);
+                        if (si < 1 || si > l) {
+                            if (!x_open) {
+                                si = wrap(si, l);
+                            } else if (si == 0) {
+                                x[q] = inl[q];              /* q = 1..5 */
+                                continue;
+                            } else {
+                                gather(a, g, IDX(g, l, sj, sk), y);
+                                outlet_eq(y, o);
+                                x[q] = o[q - 9];            /* q = 10..14 */
+                                continue;
+                            }
+                        }
+                        if (OBS(si, sj, sk) == 1)
+                            x[q] = a[OPP[q] * g.np + s];
+                        else
+                            x[q] = a[q * g.np + IDX(g, si, sj, sk)];
+                    }
+                    x[18] = a[18 * g.np + s];
+                    collide_site(x, out, omega, fgrad);
+                    for (int q = 0; q < 19; ++q) b[q * g.np + s] = out[q];
+                }
+        for (int q = 0; q < 19; ++q)
+            for (int k = 1; k <= n; ++k)
+                for (int j = 1; j <= m; ++j)
+                    memcpy(&a[q * g.np + IDX(g, 1, j, k)], &b[q * g.np + IDX(g, 1, j, k)],
+                           sizeof(double) * (size_t)l);
+    }
+    return 0;
+}
+
+/* momentum-exchange force on the solid from the post-collision state (physics semantics):
+ * F = sum over fluid sites x and directions p with x + c_p solid of 2 c_p f_p(x). */
+void orc_mem_force(const double *a, const int32_t *obs, int l, int m, int n, int x_open, double F[3])
+{
+    geom_t g = mkgeom(l, m, n);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int k = 1; k <= n; ++k)
+        for (int j = 1; j <= m; ++j)
+            for (int i = 1; i <= l; ++i) {
+                if (OBS(i, j, k) == 1) continue;
+                long s = IDX(g, i, j, k);
+                for (int p = 0; p < 18; ++p) {
+                    int ti = i + CX[p], tj = wrap(j + CY[p], m), tk = wrap(k + CZ[p], n);
+                    if (ti < 1 || ti > l) {
+                        if (x_open) continue;
+                        ti = wrap(ti, l);
+                    }
+                    if (OBS(ti, tj, tk) != 1) continue;
+                    double f2 = 2.0 * a[p * g.np + s];
+                    fx += CX[p] * f2;
+                    fy += CY[p] * f2;
+                    fz += CZ[p] * f2;
+                }
+            }
+    F[0] = fx; F[1] = fy; F[2] = fz;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* diagnostics                                                                                 */
+/* ------------------------------------------------------------------------------------------ */
+static const double CXD[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+static const double CYD[19] = {-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1, 0};
+
+static void site_fneq(const double *x, double *fn, double *rho_out)
+{
+    /* flux_X.F90:173-242 */
+    double rho = +x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9] + x[10]
+                 + x[11] + x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18];
+    double rhoinv = 1.0 / rho;
+    double xj = +(x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) * rhoinv;
+    double yj = +(x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]) * rhoinv;
+    double zj = +(x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]) * rhoinv;
+    double cvsq = xj * xj + yj * yj + zj * zj;
+    double fe[18];
+    fe[0] = rho * P2 * (1.0 + RF * (xj - yj) + QF * (3.0 * (xj - yj) * (xj - yj) - cvsq));
+    fe[1] = rho * P2 * (1.0 + RF * (xj - zj) + QF * (3.0 * (xj - zj) * (xj - zj) - cvsq));
+    fe[2] = rho * P2 * (1.0 + RF * (xj + yj) + QF * (3.0 * (xj + yj) * (xj + yj) - cvsq));
+    fe[3] = rho * P2 * (1.0 + RF * (xj + zj) + QF * (3.0 * (xj + zj) * (xj + zj) - cvsq));
+    fe[4] = rho * P1 * (1.0 + RF * (xj) + QF * (3.0 * (xj) * (xj)-cvsq));
+    fe[5] = rho * P1 * (1.0 + RF * (zj) + QF * (3.0 * (zj) * (zj)-cvsq));
+    fe[6] = rho * P2 * (1.0 + RF * (yj + zj) + QF * (3.0 * (yj + zj) * (yj + zj) - cvsq));
+    fe[7] = rho * P1 * (1.0 + RF * (yj) + QF * (3.0 * (yj) * (yj)-cvsq));
+    fe[8] = rho * P2 * (1.0 + RF * (yj - zj) + QF * (3.0 * (yj - zj) * (yj - zj) - cvsq));
+    fe[9] = rho * P2 * (1.0 + RF * (-xj - yj) + QF * (3.0 * (xj + yj) * (xj + yj) - cvsq));
+    fe[10] = rho * P2 * (1.0 + RF * (-xj - zj) + QF * (3.0 * (xj + zj) * (xj + zj) - cvsq));
+    fe[11] = rho * P2 * (1.0 + RF * (-xj + yj) + QF * (3.0 * (xj - yj) * (xj - yj) - cvsq));
+    fe[12] = rho * P2 * (1.0 + RF * (-xj + zj) + QF * (3.0 * (xj - zj) * (xj - zj) - cvsq));
+    fe[13] = rho * P1 * (1.0 + RF * (-xj) + QF * (3.0 * (xj) * (xj)-cvsq));
+    fe[14] = rho * P1 * (1.0 + RF * (-zj) + QF * (3.0 * (zj) * (zj)-cvsq));
+    fe[15] = rho * P2 * (1.0 + RF * (-yj - zj) + QF * (3.0 * (yj + zj) * (yj + zj) - cvsq));
+    fe[16] = rho * P1 * (1.0 + RF * (-yj) + QF * (3.0 * (yj) * (yj)-cvsq));
+    fe[17] = rho * P2 * (1.0 + RF * (-yj + zj) + QF * (3.0 * (yj - zj) * (yj - zj) - cvsq));
+    for (int q = 0; q < 18; ++q) fn[q] = x[q] - fe[q];
+    *rho_out = rho;
+}
+
+/* SERIAL index clipping of flux_X/flux_Y for one axis: returns 0 = ok, 1 = return early */
+typedef struct { int lo, hi; double n1, n2; int ret; } clip_t;
+
+/* flux_X.F90:50-301.  box = {xstart, xstop, ystart, ystop, zstart, zstop}; result -> *flux */
+void orc_flux_x(const double *a, int l, int m, int n, double omega, const int box[6], double *flux)
+{
+    geom_t g = mkgeom(l, m, n);
+    int istart = box[0], istop = box[1], jstart = box[2], jstop = box[3];
+    int kstart = box[4], kstop = box[5];
+    int i1 = 0, i2 = 0, j1 = 0, j2 = 0, k1 = 0, k2 = 0;
+    double norm1 = 1.0, norm2 = 1.0, fl[2] = {0.0, 0.0};
+    if (istart > 0 && istop <= l) { i1 = istart; i2 = istop; }
+    if (istart > 0 && istop > l) { i1 = istart; i2 = l; norm2 = 0.0; }
+    if (istart <= 0 && istop <= l) { i1 = 1; norm1 = 0.0; i2 = istop; }
+    if (istart < 0 && istop > l) return;
+    if (istart > l || istop < 0) return;
+    if (jstart > 0 && jstop <= m) { j1 = jstart; j2 = jstop; }
+    if (jstart > 0 && jstop > m) { j1 = jstart; j2 = m; }
+    if (jstart <= 0 && jstop <= m) { j1 = 1; j2 = jstop; }
+    if (jstart < 0 && jstop > m) { j1 = 1; j2 = m; }
+    if (jstart > m || jstop < 0) return;
+    if (kstart > 0 && kstop <= n) { k1 = kstart; k2 = kstop; }
+    if (kstart > 0 && kstop > n) { k1 = kstart; k2 = n; }
+    if (kstart <= 0 && kstop <= n) { k1 = 1; k2 = kstop; }
+    if (kstart < 0 && kstop > n) { k1 = 1; k2 = n; }
+    if (kstart > n || kstop < 0) return;
+    int face[2] = {i1, i2};
+    const double c = (1 - 0.5 * omega);
+    double x[19], fn[18], rho;
+    for (int ii = 0; ii < 2; ++ii) {
+        int i0 = face[ii];
+        for (int k = k1; k <= k2; ++k)
+            for (int j = j1; j <= j2; ++j) {
+                gather(a, g, IDX(g, i0, j, k), x);
+                site_fneq(x, fn, &rho);
+                double sxx = -rho / RF;
+                for (int q = 0; q < 18; ++q) sxx = sxx - c * CXD[q] * CXD[q] * fn[q];
+                fl[ii] = fl[ii] + sxx;
+            }
+    }
+    *flux = -fl[0] * norm1 + fl[1] * norm2;
+}
+
+/* flux_Y.F90:51-282 */
+void orc_flux_y(const double *a, int l, int m, int n, double omega, const int box[6], double *flux)
+{
+    geom_t g = mkgeom(l, m, n);
+    int istart = box[0], istop = box[1], jstart = box[2], jstop = box[3];
+    int kstart = box[4], kstop = box[5];
+    int i1 = 0, i2 = 0, j1 = 0, j2 = 0, k1 = 0, k2 = 0;
+    double norm1 = 1.0, norm2 = 1.0, fl[2] = {0.0, 0.0};
+    if (jstart > 0 && jstop <= m) { j1 = jstart; j2 = jstop; }
+    if (jstart > 0 && jstop > m) { j1 = jstart; j2 = m; norm2 = 0.0; }
+    if (jstart <= 0 && jstop <= m) { j1 = 1; norm1 = 0.0; j2 = jstop; }
+    if (jstart < 0 && jstop > m) return;
+    if (jstart > m || jstop < 0) return;
+    if (istart > 0 && istop <= l) { i1 = istart; i2 = istop; }
+    if (istart > 0 && istop > l) { i1 = istart; i2 = l; }
+    if (istart <= 0 && istop <= l) { i1 = 1; i2 = istop; }
+    if (istart < 0 && istop > l) { i1 = 1; i2 = l; }
+    if (istart > l || istop < 0) return;
+    if (kstart > 0 && kstop <= n) { k1 = kstart; k2 = kstop; }
+    if (kstart > 0 && kstop > n) { k1 = kstart; k2 = n; }
+    if (kstart <= 0 && kstop <= n) { k1 = 1; k2 = kstop; }
+    if (kstart < 0 && kstop > n) { k1 = 1; k2 = n; }
+    if (kstart > n || kstop < 0) return;
+    int face[2] = {j1, j2};
+    const double c = (1 - 0.5 * omega);
+    double x[19], fn[18], rho;
+    for (int ii = 0; ii < 2; ++ii) {
+        int j0 = face[ii];
+        for (int k = k1; k <= k2; ++k)
+            for (int i = i1; i <= i2; ++i) {
+                gather(a, g, IDX(g, i, j0, k), x);
+                site_fneq(x, fn, &rho);
+                double sxy = 0.0;
+                for (int q = 0; q < 18; ++q) sxy = sxy - c * CXD[q] * CYD[q] * fn[q];
+                fl[ii] = fl[ii] + sxy;
+            }
+    }
+    *flux = +fl[0] * norm1 + fl[1] * norm2;
+}
+
+/* probe_global.F90:41-77: out = {u, v, w, rho} at interior site (i, j, k); returns 1 if the
+ * site is outside the lattice (the reference then writes nothing) */
+int orc_probe(const double *a, int l, int m, int n, int i, int j, int k, double out[4])
+{
+    geom_t g = mkgeom(l, m, n);
+    if (!(k > 0 && k <= n && j > 0 && j <= m && i > 0 && i <= l)) return 1;
+    double x[19];
+    gather(a, g, IDX(g, i, j, k), x);
+    double rho = x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9] + x[10]
+                 + x[11] + x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18];
+    double xj = (x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]);
+    double yj = (x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]);
+    double zj = (x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]);
+    out[0] = xj / rho;
+    out[1] = yj / rho;
+    out[2] = zj / rho;
+    out[3] = rho;
+    return 0;
+}
+
+/* diagno.F90:53-131 (SERIAL): out = {<rho>, <u>, <v>, <w>, <u^2>} */
+void orc_diagno(const double *a, const int32_t *obs, int l, int m, int n, double out[5])
+{
+    geom_t g = mkgeom(l, m, n);
+    double rdv = 1. / ((double)l * (double)m * (double)n);
+    double rtot = 0., xtot = 0., ytot = 0., ztot = 0., stot = 0.;
+    double x[19];
+    for (int k = 1; k <= n; ++k)
+        for (int j = 1; j <= m; ++j)
+            for (int i = 1; i <= l; ++i) {
+                double rho, xj, yj, zj;
+                if (OBS(i, j, k) == 1) {
+                    rho = 1.0; xj = 0.0; yj = 0.0; zj = 0.0;
+                } else {
+                    gather(a, g, IDX(g, i, j, k), x);
+                    rho = +x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9]
+                          + x[10] + x[11] + x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18];
+                    double rhoinv = 1.0 / rho;
+                    xj = +(x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) * rhoinv;
+                    yj = +(x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]) * rhoinv;
+                    zj = +(x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]) * rhoinv;
+                }
+                rtot = rtot + rho;
+                xtot = xtot + xj;
+                ytot = ytot + yj;
+                ztot = ztot + zj;
+                stot = stot + xj * xj + yj * yj + zj * zj;
+            }
+    out[0] = rtot * rdv;
+    out[1] = xtot * rdv;
+    out[2] = ytot * rdv;
+    out[3] = ztot * rdv;
+    out[4] = stot * rdv;
+}

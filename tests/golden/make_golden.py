@@ -1,0 +1,110 @@
+"""Generate the golden fixtures in tests/golden/ from the reference solver itself.
+
+Runs the reference's serial FP64 executable (built from its own sources by oracle/Makefile.ref)
+on inputs written by this repository's seeded generators, and records:
+
+* SHA-256 of the generated inputs (obstacle mask as int32 (l,m,n), initial ghosted populations) so
+  a drift of the generators is detected;
+* SHA-256 of the full ghosted population state saved by the reference (save_raw) after given
+  step counts, and for the smallest case the whole final state (``*.npz``);
+* the reference's probe records (probe_g.0000.dat, e14.6 text) and drag records (drag.0000.dat,
+  17-digit doubles) where the case enables them.
+
+Usage (in a container where /root/reference is present):
+    make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from euplectella_hpc_amd import geometry  # noqa: E402
+from oracle import refrun  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+# name: lattice, geometry, tau, u_in, seed, steps, save points, probe/drag options
+CASES = {
+    "c0_tiny": dict(l=16, m=8, n=12, geom=("cylinder", dict(radius=2.5, centre=(5, 4.5))),
+                    tau=0.6, u=0.05, seed=11, steps=20, saves=(10, 20), probe=True,
+                    full_state=True),
+    "c1_small": dict(l=32, m=16, n=16, geom=("cylinder", dict(radius=4, centre=(8, 8))),
+                     tau=0.6, u=0.05, seed=1, steps=10, saves=(5, 10), probe=True),
+    "c1_cylinder": dict(l=128, m=64, n=64, geom=("cylinder", dict(radius=8, centre=(32, 32))),
+                        tau=0.6, u=0.05, seed=2024, steps=2000, saves=(200, 1000, 2000),
+                        probe=True),
+    "c2_ridged": dict(l=64, m=32, n=32,
+                      geom=("ridged_cylinder", dict(radius=6, centre=(16, 16), n_ridges=8,
+                                                    width=2)),
+                      tau=0.52, u=0.08, seed=3, steps=200, saves=(100, 200), probe=True),
+    "c3_sponge": dict(l=96, m=48, n=48,
+                      geom=("sponge", dict(radius=12, centre=(24, 24), cell=8, thickness=3.0,
+                                           seed=4)),
+                      tau=0.505, u=0.06, seed=4, steps=200, saves=(100, 200), probe=True),
+    "drag_r100": dict(l=680, m=340, n=4, geom=("cylinder", dict(radius=100, centre=(170, 170))),
+                      tau=0.6, u=0.05, seed=5, steps=3, saves=(3,), probe=False, drag=True),
+}
+
+
+def sha(arr):
+    return hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest()
+
+
+def make_inputs(case):
+    l, m, n = case["l"], case["m"], case["n"]
+    kind, kw = case["geom"]
+    mask = getattr(geometry, kind)(l, m, n, **kw)
+    pops = geometry.initial_state(l, m, n, case["u"], seed=case["seed"])
+    return mask, pops
+
+
+def main(names=None):
+    index = {}
+    for name, case in CASES.items():
+        if names and name not in names:
+            continue
+        l, m, n = case["l"], case["m"], case["n"]
+        mask, pops = make_inputs(case)
+        svisc = (2.0 * case["tau"] - 1.0) / 6.0
+        drag = case.get("drag", False)
+        saves = case["saves"]
+        step = saves[0] if len(saves) == 1 else int(np.gcd.reduce(np.array(saves)))
+        res = refrun.run_reference(l, m, n, svisc, case["u"], case["steps"], mask, pops,
+                                   variant="serial" if drag else "nodrag", itsave=step,
+                                   icheck=1 if (drag or case.get("probe")) else None)
+        rec = {
+            "l": l, "m": m, "n": n, "geometry": [case["geom"][0], case["geom"][1]],
+            "tau": case["tau"], "svisc": svisc, "u_in": case["u"], "seed": case["seed"],
+            "steps": case["steps"], "nsolid": int(mask.sum()),
+            "mask_sha256": sha(mask.astype(np.int32)),
+            "init_sha256": sha(pops),
+            "state_sha256": {str(t): sha(res["saves"][t]) for t in saves},
+        }
+        if case.get("probe"):
+            rec["probe_site"] = [8 * l // 20, 9 * m // 16, n // 2 - 5]
+            rec["probe"] = [list(r) for r in res["probe"]]
+        if drag:
+            rec["drag"] = [list(r[:4]) for r in res["drag"]]
+        if case.get("full_state"):
+            last = saves[-1]
+            np.savez_compressed(os.path.join(OUT, f"{name}_state.npz"),
+                                **{f"t{last}": res["saves"][last]})
+        index[name] = rec
+        print(name, "done", rec["state_sha256"], flush=True)
+    path = os.path.join(OUT, "golden.json")
+    old = {}
+    if names and os.path.exists(path):
+        with open(path) as fh:
+            old = json.load(fh)
+    old.update(index)
+    with open(path, "w") as fh:
+        json.dump(old, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or None)

@@ -1,0 +1,70 @@
+"""The C oracle against the golden fixtures produced by the reference executable."""
+import os
+
+import numpy as np
+import pytest
+
+from golden_cases import CASES, make_inputs, omega_of, sha, e14_6
+from oracle import oracle as orc
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_generated_inputs_match_fixture(golden, name):
+    rec = golden[name]
+    mask, pops = make_inputs(CASES[name])
+    assert int(mask.sum()) == rec["nsolid"]
+    assert sha(mask.astype(np.int32)) == rec["mask_sha256"]
+    assert sha(pops) == rec["init_sha256"]
+
+
+def test_oracle_tiny_full_state(golden):
+    rec = golden["c0_tiny"]
+    mask, pops = make_inputs(CASES["c0_tiny"])
+    orc.parity_steps(pops, mask, omega_of(rec), rec["u_in"], rec["steps"])
+    ref = np.load(os.path.join(GOLDEN, "c0_tiny_state.npz"))[f"t{rec['steps']}"]
+    assert ref.shape == pops.shape
+    assert np.array_equal(pops.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("name,upto", [("c1_small", 10), ("c2_ridged", 200), ("c3_sponge", 200),
+                                       ("c1_cylinder", 200)])
+def test_oracle_state_hashes(golden, name, upto):
+    rec = golden[name]
+    mask, pops = make_inputs(CASES[name])
+    done = 0
+    for t in sorted(int(s) for s in rec["state_sha256"]):
+        if t > upto:
+            break
+        orc.parity_steps(pops, mask, omega_of(rec), rec["u_in"], t - done)
+        done = t
+        assert sha(pops) == rec["state_sha256"][str(t)], f"{name} differs at step {t}"
+    assert done > 0
+
+
+@pytest.mark.parametrize("name", ["c0_tiny", "c1_small", "c2_ridged"])
+def test_oracle_probe_records(golden, name):
+    rec = golden[name]
+    mask, pops = make_inputs(CASES[name])
+    i, j, k = rec["probe_site"]
+    for row in rec["probe"][:50]:
+        orc.parity_steps(pops, mask, omega_of(rec), rec["u_in"], 1)
+        got = orc.probe(pops, i, j, k)
+        assert int(row[0]) > 0
+        assert [e14_6(v) for v in got] == [e14_6(v) for v in row[1:5]], f"step {row[0]}"
+
+
+def test_oracle_drag_records(golden):
+    rec = golden["drag_r100"]
+    mask, pops = make_inputs(CASES["drag_r100"])
+    om = omega_of(rec)
+    rows = iter(rec["drag"])
+    for step in range(1, rec["steps"] + 1):
+        orc.parity_steps(pops, mask, om, rec["u_in"], 1)
+        for delta, box in orc.drag_boxes(rec["l"], rec["m"], rec["n"]):
+            d, t, fx, fy = next(rows)
+            assert (int(d), int(t)) == (delta, step)
+            assert orc.flux_x(pops, om, box) == fx
+            assert orc.flux_y(pops, om, box) == fy
+    assert sha(pops) == rec["state_sha256"][str(rec["steps"])]
