@@ -1,0 +1,45 @@
+"""Build libeup.so (HIP, gfx950) in-tree with hipcc.
+
+Compile flags:
+  --offload-arch=gfx950       MI355X only
+  -ffp-contract=off           no fused multiply-add: the FP64 paths are bit-identical to the
+                              reference's serial build
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                              no packed FP32 instructions in the generated code
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SOURCES = ["eup_api.hip", "eup_parity.hip", "eup_physics.hip", "eup_diag.hip"]
+OUT = os.path.join(HERE, "libeup.so")
+
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+               "-ffp-contract=off", "-Wno-unused-result", "-Wno-unused-value",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.isabs(cand) and os.path.exists(cand) or not os.path.isabs(cand)):
+            return cand
+    return "hipcc"
+
+
+def build(verbose=False):
+    srcs = [os.path.join(HERE, "csrc", s) for s in SOURCES]
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include")] + srcs + ["-o", OUT + ".tmp"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building libeup.so")
+    os.replace(OUT + ".tmp", OUT)
+    if verbose:
+        print("built", OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,401 @@
+// extern "C" entry points of libeup.so (declared in include/eup.h).
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+
+#include "eup_internal.h"
+
+namespace eup {
+
+static thread_local std::string g_last_error;
+
+int fail(Handle *h, int code, const std::string &msg)
+{
+    if (h) h->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+int hip_check(Handle *h, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return 0;
+    std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+    fail(h, EUP_EHIP, msg);
+    return EUP_EHIP;
+}
+
+static size_t storage_bytes(int storage)
+{
+    switch (storage) {
+    case EUP_STORE_F64: return 8;
+    case EUP_STORE_F32: return 4;
+    case EUP_STORE_F16: return 2;
+    case EUP_STORE_BF16: return 2;
+    }
+    return 0;
+}
+
+static int validate(const eup_config *c, std::string &why)
+{
+    char buf[256];
+    if (c->l < 2 || c->m < 2 || c->n < 2) {
+        snprintf(buf, sizeof buf, "lattice %d x %d x %d: every size must be >= 2", c->l, c->m, c->n);
+        why = buf;
+        return EUP_EINVAL;
+    }
+    const double ghosted = (double)(c->l + 2) * (c->m + 2) * (c->n + 2);
+    if (ghosted >= 2147483647.0) {
+        why = "lattice too large: (l+2)(m+2)(n+2) must stay below 2^31 sites";
+        return EUP_EINVAL;
+    }
+    if (c->mode != EUP_MODE_PARITY && c->mode != EUP_MODE_PHYSICS) {
+        why = "mode must be EUP_MODE_PARITY or EUP_MODE_PHYSICS";
+        return EUP_EINVAL;
+    }
+    if (storage_bytes(c->storage) == 0) {
+        why = "unknown storage";
+        return EUP_EINVAL;
+    }
+    if (c->up != EUP_FACE_PERIODIC || c->down != EUP_FACE_PERIODIC || c->left != EUP_FACE_PERIODIC ||
+        c->right != EUP_FACE_PERIODIC) {
+        why = "y and z faces must be periodic (CYLINDER_INF face flags, build.bcond.F90:319-339)";
+        return EUP_EINVAL;
+    }
+    const bool open = c->front == EUP_FACE_OUTFLOW && c->rear == EUP_FACE_INFLOW;
+    const bool per = c->front == EUP_FACE_PERIODIC && c->rear == EUP_FACE_PERIODIC;
+    if (c->mode == EUP_MODE_PARITY) {
+        if (c->storage != EUP_STORE_F64) {
+            why = "parity mode requires EUP_STORE_F64";
+            return EUP_EINVAL;
+        }
+        if (!open) {
+            why = "parity mode requires front = outflow (5) and rear = inflow (4)";
+            return EUP_EINVAL;
+        }
+    } else if (!open && !per) {
+        why = "x faces must be outflow/inflow (5/4) or both periodic (0/0)";
+        return EUP_EINVAL;
+    }
+    if (!(c->omega > 0.0 && c->omega < 2.0)) {
+        why = "omega must lie in (0, 2)";
+        return EUP_EINVAL;
+    }
+    return 0;
+}
+
+} // namespace eup
+
+using namespace eup;
+
+struct eup_handle : Handle {};
+
+static void release(Handle *h)
+{
+    if (!h) return;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    void *ptrs[] = {h->buf[0], h->buf[1], h->staging, h->obs, h->solid, h->solid_list, h->bb_tmp,
+                    h->tile_code, h->link_mask, h->mixed_list, h->outlet[0], h->outlet[1], h->diag,
+                    h->site19};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->stream) hipStreamDestroy(h->stream);
+}
+
+extern "C" {
+
+int32_t eup_abi_version(void) { return EUP_ABI_VERSION; }
+
+const char *eup_last_error(const eup_handle *h)
+{
+    if (h) return h->err.c_str();
+    return g_last_error.c_str();
+}
+
+int eup_create(const eup_config *cfg, eup_handle **out)
+{
+    if (!cfg || !out) return fail(nullptr, EUP_EINVAL, "eup_create: null argument");
+    *out = nullptr;
+    std::string why;
+    if (validate(cfg, why)) return fail(nullptr, EUP_EINVAL, "eup_create: " + why);
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0)
+        return fail(nullptr, EUP_EHIP, "eup_create: no HIP device available");
+    if (cfg->device < 0 || cfg->device >= ndev)
+        return fail(nullptr, EUP_EINVAL, "eup_create: device ordinal out of range");
+    eup_handle *h = new (std::nothrow) eup_handle();
+    if (!h) return fail(nullptr, EUP_EHIP, "eup_create: out of host memory");
+    h->cfg = *cfg;
+    Geometry &g = h->g;
+    g.l = cfg->l;
+    g.m = cfg->m;
+    g.n = cfg->n;
+    g.sites = (int64_t)g.l * g.m * g.n;
+    g.ghosted = (int64_t)(g.l + 2) * (g.m + 2) * (g.n + 2);
+    g.tiles_per_row = (g.l + kTile - 1) / kTile;
+    g.tiles = (int64_t)g.tiles_per_row * g.m * g.n;
+    h->elem = storage_bytes(cfg->storage);
+    const int64_t per = cfg->mode == EUP_MODE_PARITY ? g.ghosted : g.sites;
+    h->stride = (per + 255) / 256 * 256;
+    int rc = 0;
+    if (hip_check(h, hipSetDevice(cfg->device), "hipSetDevice") ||
+        hip_check(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate") ||
+        hip_check(h, hipEventCreate(&h->ev0), "hipEventCreate") ||
+        hip_check(h, hipEventCreate(&h->ev1), "hipEventCreate"))
+        rc = EUP_EHIP;
+    const size_t state = h->elem * (size_t)h->stride * 19;
+    for (int b = 0; b < 2 && !rc; ++b)
+        if (hip_check(h, hipMalloc(&h->buf[b], state), "hipMalloc populations")) rc = EUP_EHIP;
+    if (!rc && (hip_check(h, hipMalloc((void **)&h->staging, sizeof(double) * g.ghosted), "hipMalloc staging") ||
+                hip_check(h, hipMalloc((void **)&h->obs, sizeof(int32_t) * g.sites), "hipMalloc obs") ||
+                hip_check(h, hipMalloc((void **)&h->solid, g.sites), "hipMalloc solid")))
+        rc = EUP_EHIP;
+    if (!rc && cfg->mode == EUP_MODE_PHYSICS) {
+        const size_t rb = cfg->storage == EUP_STORE_F64 ? 8 : 4;
+        for (int b = 0; b < 2 && !rc; ++b)
+            if (hip_check(h, hipMalloc(&h->outlet[b], rb * 5 * (size_t)g.m * g.n), "hipMalloc outlet")) rc = EUP_EHIP;
+    }
+    if (rc) {
+        g_last_error = "eup_create: " + h->err;
+        release(h);
+        delete h;
+        return rc;
+    }
+    h->device_bytes = (int64_t)(2 * state + sizeof(double) * g.ghosted + 5 * g.sites);
+    *out = h;
+    return 0;
+}
+
+void eup_destroy(eup_handle *h)
+{
+    if (!h) return;
+    hipSetDevice(h->cfg.device);
+    release(h);
+    delete h;
+}
+
+int eup_set_mask(eup_handle *h, const int32_t *obs)
+{
+    if (!h || !obs) return fail(h, EUP_EINVAL, "eup_set_mask: null argument");
+    if (hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice")) return EUP_EHIP;
+    const Geometry &g = h->g;
+    // validate values and build the dense solid flags on the host (setup-time work)
+    uint8_t *sol = nullptr;
+    if (hip_check(h, hipHostMalloc((void **)&sol, g.sites), "hipHostMalloc")) return EUP_EHIP;
+    int64_t ns = 0;
+    int bb[6] = {g.l, 0, g.m, 0, g.n, 0};
+    for (int k = 1; k <= g.n; ++k)
+        for (int j = 1; j <= g.m; ++j)
+            for (int i = 1; i <= g.l; ++i) {
+                const int64_t o = (int64_t)(i - 1) + (int64_t)g.l * ((int64_t)(j - 1) + (int64_t)g.m * (k - 1));
+                const int32_t v = obs[o];
+                if (v != 0 && v != 1) {
+                    hipHostFree(sol);
+                    return fail(h, EUP_EINVAL, "eup_set_mask: obs values must be 0 or 1");
+                }
+                sol[o] = (uint8_t)v;
+                if (v == 1) {
+                    ++ns;
+                    if (i < bb[0]) bb[0] = i;
+                    if (i > bb[1]) bb[1] = i;
+                    if (j < bb[2]) bb[2] = j;
+                    if (j > bb[3]) bb[3] = j;
+                    if (k < bb[4]) bb[4] = k;
+                    if (k > bb[5]) bb[5] = k;
+                }
+            }
+    int rc = 0;
+    if (hip_check(h, hipMemcpy(h->obs, obs, sizeof(int32_t) * g.sites, hipMemcpyHostToDevice), "copy obs") ||
+        hip_check(h, hipMemcpy(h->solid, sol, g.sites, hipMemcpyHostToDevice), "copy solid"))
+        rc = EUP_EHIP;
+    hipHostFree(sol);
+    if (rc) return rc;
+    h->nsolid = ns;
+    memcpy(h->bbox, bb, sizeof bb);
+    rc = h->cfg.mode == EUP_MODE_PARITY ? parity_prepare_mask(h) : physics_prepare_mask(h);
+    if (rc) return rc;
+    h->mask_set = true;
+    return 0;
+}
+
+static int check_q(eup_handle *h, int32_t q)
+{
+    if (q < 1 || q > 19) return fail(h, EUP_ERANGE, "population index must be 1..19");
+    return 0;
+}
+
+int eup_set_pops(eup_handle *h, int32_t q, const double *a)
+{
+    if (!h || !a) return fail(h, EUP_EINVAL, "eup_set_pops: null argument");
+    if (check_q(h, q)) return EUP_ERANGE;
+    if (hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice") ||
+        hip_check(h, hipMemcpyAsync(h->staging, a, sizeof(double) * h->g.ghosted, hipMemcpyHostToDevice,
+                                    h->stream), "copy pops"))
+        return EUP_EHIP;
+    int rc = h->cfg.mode == EUP_MODE_PARITY ? parity_set_pop(h, q) : physics_set_pop(h, q);
+    if (rc) return rc;
+    if (hip_check(h, hipStreamSynchronize(h->stream), "sync")) return EUP_EHIP;
+    h->pops_set |= 1u << (q - 1);
+    return 0;
+}
+
+int eup_get_pops(eup_handle *h, int32_t q, double *a)
+{
+    if (!h || !a) return fail(h, EUP_EINVAL, "eup_get_pops: null argument");
+    if (check_q(h, q)) return EUP_ERANGE;
+    if (hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice")) return EUP_EHIP;
+    int rc = h->cfg.mode == EUP_MODE_PARITY ? parity_get_pop(h, q) : physics_get_pop(h, q);
+    if (rc) return rc;
+    if (hip_check(h, hipMemcpyAsync(a, h->staging, sizeof(double) * h->g.ghosted, hipMemcpyDeviceToHost,
+                                    h->stream), "copy pops") ||
+        hip_check(h, hipStreamSynchronize(h->stream), "sync"))
+        return EUP_EHIP;
+    return 0;
+}
+
+int eup_set_state(eup_handle *h, const double *a)
+{
+    if (!h || !a) return fail(h, EUP_EINVAL, "eup_set_state: null argument");
+    for (int q = 1; q <= 19; ++q) {
+        int rc = eup_set_pops(h, q, a + (int64_t)(q - 1) * h->g.ghosted);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int eup_get_state(eup_handle *h, double *a)
+{
+    if (!h || !a) return fail(h, EUP_EINVAL, "eup_get_state: null argument");
+    for (int q = 1; q <= 19; ++q) {
+        int rc = eup_get_pops(h, q, a + (int64_t)(q - 1) * h->g.ghosted);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+static int ready(eup_handle *h, const char *who)
+{
+    if (!h) return fail(nullptr, EUP_EINVAL, std::string(who) + ": null handle");
+    if (!h->mask_set) return fail(h, EUP_ESTATE, std::string(who) + ": eup_set_mask has not been called");
+    if (h->pops_set != 0x7FFFFu)
+        return fail(h, EUP_ESTATE, std::string(who) + ": not all 19 populations have been set");
+    if (hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice")) return EUP_EHIP;
+    return 0;
+}
+
+static int do_steps(eup_handle *h, int32_t nsteps)
+{
+    for (int32_t t = 0; t < nsteps; ++t) {
+        int rc = h->cfg.mode == EUP_MODE_PARITY ? parity_step(h) : physics_step(h);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int eup_step(eup_handle *h, int32_t nsteps)
+{
+    int rc = ready(h, "eup_step");
+    if (rc) return rc;
+    if (nsteps < 0) return fail(h, EUP_EINVAL, "eup_step: nsteps < 0");
+    rc = do_steps(h, nsteps);
+    if (rc) return rc;
+    return hip_check(h, hipStreamSynchronize(h->stream), "eup_step") ? EUP_EHIP : 0;
+}
+
+int eup_step_timed(eup_handle *h, int32_t nsteps, double *gpu_ms)
+{
+    int rc = ready(h, "eup_step_timed");
+    if (rc) return rc;
+    if (nsteps < 0 || !gpu_ms) return fail(h, EUP_EINVAL, "eup_step_timed: bad argument");
+    if (h->cfg.mode == EUP_MODE_PHYSICS && h->outlet_dirty) {
+        rc = physics_refresh_outlet(h);
+        if (rc) return rc;
+    }
+    if (hip_check(h, hipEventRecord(h->ev0, h->stream), "hipEventRecord")) return EUP_EHIP;
+    rc = do_steps(h, nsteps);
+    if (rc) return rc;
+    float ms = 0.f;
+    if (hip_check(h, hipEventRecord(h->ev1, h->stream), "hipEventRecord") ||
+        hip_check(h, hipEventSynchronize(h->ev1), "hipEventSynchronize") ||
+        hip_check(h, hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime"))
+        return EUP_EHIP;
+    *gpu_ms = ms;
+    return 0;
+}
+
+int eup_sync(eup_handle *h)
+{
+    if (!h) return fail(nullptr, EUP_EINVAL, "eup_sync: null handle");
+    return hip_check(h, hipStreamSynchronize(h->stream), "eup_sync") ? EUP_EHIP : 0;
+}
+
+int eup_drag_cv(eup_handle *h, const int32_t box[6], double *fluxX, double *fluxY)
+{
+    int rc = ready(h, "eup_drag_cv");
+    if (rc) return rc;
+    if (!box || !fluxX || !fluxY) return fail(h, EUP_EINVAL, "eup_drag_cv: null argument");
+    return diag_drag_cv(h, box, fluxX, fluxY);
+}
+
+int eup_drag_mem(eup_handle *h, double F[3])
+{
+    int rc = ready(h, "eup_drag_mem");
+    if (rc) return rc;
+    if (!F) return fail(h, EUP_EINVAL, "eup_drag_mem: null argument");
+    return diag_drag_mem(h, F);
+}
+
+int eup_probe(eup_handle *h, int32_t i, int32_t j, int32_t k, double out[4])
+{
+    int rc = ready(h, "eup_probe");
+    if (rc) return rc;
+    if (!out) return fail(h, EUP_EINVAL, "eup_probe: null argument");
+    if (i < 1 || i > h->g.l || j < 1 || j > h->g.m || k < 1 || k > h->g.n)
+        return fail(h, EUP_ERANGE, "eup_probe: site outside the lattice");
+    double x[19];
+    rc = diag_site(h, i, j, k, x);
+    if (rc) return rc;
+    // probe_global.F90:60-77
+    const double rho = x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9] + x[10]
+                       + x[11] + x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18];
+    const double xj = (x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]);
+    const double yj = (x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]);
+    const double zj = (x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]);
+    out[0] = xj / rho;
+    out[1] = yj / rho;
+    out[2] = zj / rho;
+    out[3] = rho;
+    return 0;
+}
+
+int eup_diagno(eup_handle *h, double out[5])
+{
+    int rc = ready(h, "eup_diagno");
+    if (rc) return rc;
+    if (!out) return fail(h, EUP_EINVAL, "eup_diagno: null argument");
+    return diag_diagno(h, out);
+}
+
+int eup_get_info(eup_handle *h, eup_info *info)
+{
+    if (!h || !info) return fail(h, EUP_EINVAL, "eup_get_info: null argument");
+    memset(info, 0, sizeof *info);
+    info->sites = h->g.sites;
+    info->nsolid = h->nsolid;
+    info->tiles = h->g.tiles;
+    info->tiles_fluid = h->tiles_fluid;
+    info->tiles_mixed = h->tiles_mixed;
+    info->tiles_solid = h->tiles_solid;
+    memcpy(info->bbox, h->bbox, sizeof h->bbox);
+    info->bytes_per_site = (int32_t)(2 * 19 * h->elem);
+    info->device_bytes = h->device_bytes;
+    return 0;
+}
+
+void *eup_stream(eup_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+} // extern "C"

@@ -1,0 +1,91 @@
+// Internal state of a libeup handle and the launchers shared between the translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/eup.h"
+#include "eup_lattice.h"
+
+namespace eup {
+
+constexpr int kTile = 64;     // sites per tile = one wavefront along x
+constexpr int kBlock = 256;   // threads per block in the site kernels (4 tiles)
+
+// tile codes of the physics layout
+constexpr int32_t kTileFluid = -1; // no solid site in the tile or adjacent to it
+constexpr int32_t kTileSolid = -2; // every site of the tile is solid: skipped
+
+struct Geometry {
+    int l = 0, m = 0, n = 0;
+    int64_t sites = 0;            // l*m*n
+    int64_t ghosted = 0;          // (l+2)(m+2)(n+2)
+    int tiles_per_row = 0;        // ceil(l / 64)
+    int64_t tiles = 0;            // tiles_per_row * m * n
+};
+
+struct Handle {
+    eup_config cfg{};
+    Geometry g;
+    std::string err;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t elem = 8;              // bytes per stored population value
+    int64_t stride = 0;           // elements between populations in a state buffer
+    void *buf[2] = {nullptr, nullptr}; // ping-pong population buffers
+    int cur = 0;                  // buf[cur] holds the current state
+    double *staging = nullptr;    // ghosted FP64 staging array (one population)
+    bool mask_set = false;
+    uint32_t pops_set = 0;        // bit q-1 set once population q has been provided
+    // mask derived data
+    int32_t *obs = nullptr;       // device copy of obs (l,m,n)
+    uint8_t *solid = nullptr;     // dense solid flags (l*m*n)
+    int64_t nsolid = 0;
+    int32_t bbox[6] = {0, 0, 0, 0, 0, 0};
+    // parity mode
+    int32_t *solid_list = nullptr; // ghosted linear offsets of solid sites
+    double *bb_tmp = nullptr;      // nsolid x 18 bounce-back values
+    // physics mode
+    int32_t *tile_code = nullptr;  // per tile: kTileFluid, kTileSolid or index into link_mask
+    uint32_t *link_mask = nullptr; // per mixed tile: 64 link masks
+    int64_t tiles_fluid = 0, tiles_mixed = 0, tiles_solid = 0;
+    int32_t *mixed_list = nullptr; // tile indices of mixed tiles
+    void *outlet[2] = {nullptr, nullptr}; // outlet ghost plane, 5 x m x n (compute type)
+    bool outlet_dirty = true;
+    // diagnostics scratch
+    double *diag = nullptr;       // device scratch for reductions
+    int64_t diag_len = 0;
+    double *site19 = nullptr;     // one site's 19 populations in FP64
+    int64_t device_bytes = 0;
+};
+
+int fail(Handle *h, int code, const std::string &msg);
+int hip_check(Handle *h, hipError_t e, const char *what);
+
+// parity mode (eup_parity.hip)
+int parity_prepare_mask(Handle *h);
+int parity_step(Handle *h);
+
+// physics mode (eup_physics.hip)
+int physics_prepare_mask(Handle *h);
+int physics_step(Handle *h);
+int physics_refresh_outlet(Handle *h);
+int physics_set_pop(Handle *h, int q); // staging (ghosted f64) -> both buffers
+int physics_get_pop(Handle *h, int q); // current buffer -> staging
+int parity_set_pop(Handle *h, int q);
+int parity_get_pop(Handle *h, int q);
+
+// diagnostics (eup_diag.hip)
+int diag_site(Handle *h, int i, int j, int k, double out[19]);
+int diag_drag_cv(Handle *h, const int32_t box[6], double *fx, double *fy);
+int diag_drag_mem(Handle *h, double F[3]);
+int diag_diagno(Handle *h, double out[5]);
+
+inline int64_t ghosted_index(const Geometry &g, int i, int j, int k)
+{
+    return (int64_t)i + (int64_t)(g.l + 2) * ((int64_t)j + (int64_t)(g.m + 2) * (int64_t)k);
+}
+
+} // namespace eup

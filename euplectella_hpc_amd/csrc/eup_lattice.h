@@ -1,0 +1,229 @@
+// D3Q19 lattice arithmetic shared by the HIP kernels (host- and device-callable).
+//
+// Direction numbering and weights: CODE/SRC/bgk3d.F90:31-49, storage.F90:77-81.  The collision,
+// inlet and outlet formulas are written in the reference's evaluation order (col.F90:127-229,
+// bcond.bc.F90:138-164 and :300-314); compiled with -ffp-contract=off the FP64 instantiation is
+// bit-identical to the reference's serial path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace eup {
+
+#define EUP_HD __host__ __device__ __forceinline__
+
+// Direction table as compile-time constants, 0-based population index Q = 0..18 for the
+// reference's directions 1..19; opp = index of -c_Q.
+template <int Q> struct Dir;
+#define EUP_DIR(Q, X, Y, Z, O)                                                                   \
+    template <> struct Dir<Q> {                                                                  \
+        static constexpr int cx = X, cy = Y, cz = Z, opp = O;                                    \
+    };
+EUP_DIR(0, 1, -1, 0, 11)
+EUP_DIR(1, 1, 0, -1, 12)
+EUP_DIR(2, 1, 1, 0, 9)
+EUP_DIR(3, 1, 0, 1, 10)
+EUP_DIR(4, 1, 0, 0, 13)
+EUP_DIR(5, 0, 0, 1, 14)
+EUP_DIR(6, 0, 1, 1, 15)
+EUP_DIR(7, 0, 1, 0, 16)
+EUP_DIR(8, 0, 1, -1, 17)
+EUP_DIR(9, -1, -1, 0, 2)
+EUP_DIR(10, -1, 0, -1, 3)
+EUP_DIR(11, -1, 1, 0, 0)
+EUP_DIR(12, -1, 0, 1, 1)
+EUP_DIR(13, -1, 0, 0, 4)
+EUP_DIR(14, 0, 0, -1, 5)
+EUP_DIR(15, 0, -1, -1, 6)
+EUP_DIR(16, 0, -1, 0, 7)
+EUP_DIR(17, 0, -1, 1, 8)
+EUP_DIR(18, 0, 0, 0, 18)
+#undef EUP_DIR
+
+template <typename R> struct W {
+    static constexpr R p0 = R(1) / R(3);
+    static constexpr R p1 = R(1) / R(18);
+    static constexpr R p2 = R(1) / R(36);
+};
+
+template <typename R> EUP_HD constexpr R weight(int q)
+{
+    return q == 18 ? W<R>::p0 : ((q == 4 || q == 5 || q == 7 || q == 13 || q == 14 || q == 16)
+                                     ? W<R>::p1
+                                     : W<R>::p2);
+}
+
+// BGK collision of one site, x[19] pre-collision -> out[19] (may alias); col.F90:127-229
+template <typename R>
+EUP_HD void collide(const R *x, R *out, R omega, R fgrad)
+{
+    const R one = R(1), three = R(3), rf = R(3), qf = R(1.5);
+    const R omega1 = one - omega;
+    R rho = x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9] + x[10] + x[11]
+            + x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18];
+    R rhoinv = one / rho;
+    R vx = (x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) * rhoinv;
+    R vy = (x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]) * rhoinv;
+    R vz = (x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]) * rhoinv;
+    R vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+    R vsq = vx2 + vy2 + vz2;
+    R vxpy = vx + vy;
+    R qxpy = one + qf * (three * vxpy * vxpy - vsq);
+    R vxmy = vx - vy;
+    R qxmy = one + qf * (three * vxmy * vxmy - vsq);
+    R vxpz = vx + vz;
+    R qxpz = one + qf * (three * vxpz * vxpz - vsq);
+    R vxmz = vx - vz;
+    R qxmz = one + qf * (three * vxmz * vxmz - vsq);
+    R vypz = vy + vz;
+    R qypz = one + qf * (three * vypz * vypz - vsq);
+    R vymz = vy - vz;
+    R qymz = one + qf * (three * vymz * vymz - vsq);
+    R qx = one + qf * (three * vx2 - vsq);
+    R qy = one + qf * (three * vy2 - vsq);
+    R qz = one + qf * (three * vz2 - vsq);
+    vx = rf * vx;
+    vy = rf * vy;
+    vz = rf * vz;
+    vxpy = rf * vxpy;
+    vxmy = rf * vxmy;
+    vxpz = rf * vxpz;
+    vxmz = rf * vxmz;
+    vypz = rf * vypz;
+    vymz = rf * vymz;
+    const R rp1 = rho * W<R>::p1;
+    const R rp2 = rho * W<R>::p2;
+    const R fx = fgrad * rho; // FORCING_X (col.F90:173-187)
+    const R e0 = rp2 * (vxmy + qxmy);
+    const R e1 = rp2 * (vxmz + qxmz);
+    const R e2 = rp2 * (vxpy + qxpy);
+    const R e3 = rp2 * (vxpz + qxpz);
+    const R e4 = rp1 * (vx + qx);
+    const R e5 = rp1 * (vz + qz);
+    const R e6 = rp2 * (vypz + qypz);
+    const R e7 = rp1 * (vy + qy);
+    const R e8 = rp2 * (vymz + qymz);
+    const R e9 = rp2 * (-vxpy + qxpy);
+    const R e10 = rp2 * (-vxpz + qxpz);
+    const R e11 = rp2 * (-vxmy + qxmy);
+    const R e12 = rp2 * (-vxmz + qxmz);
+    const R e13 = rp1 * (-vx + qx);
+    const R e14 = rp1 * (-vz + qz);
+    const R e15 = rp2 * (-vypz + qypz);
+    const R e16 = rp1 * (-vy + qy);
+    const R e17 = rp2 * (-vymz + qymz);
+    const R e18 = rho * W<R>::p0 * (one - qf * vsq);
+    const R z = R(0);
+    // (omega*e + fx) -/+ fy -/+ fz with fy = fz = 0, exactly as the reference's expressions
+    out[0] = omega1 * x[0] + (omega * e0 + fx - z);
+    out[1] = omega1 * x[1] + (omega * e1 + fx - z);
+    out[2] = omega1 * x[2] + (omega * e2 + fx + z);
+    out[3] = omega1 * x[3] + (omega * e3 + fx + z);
+    out[4] = omega1 * x[4] + (omega * e4 + fx);
+    out[5] = omega1 * x[5] + (omega * e5 + z);
+    out[6] = omega1 * x[6] + (omega * e6 + z + z);
+    out[7] = omega1 * x[7] + (omega * e7 + z);
+    out[8] = omega1 * x[8] + (omega * e8 + z - z);
+    out[9] = omega1 * x[9] + (omega * e9 - fx - z);
+    out[10] = omega1 * x[10] + (omega * e10 - fx - z);
+    out[11] = omega1 * x[11] + (omega * e11 - fx + z);
+    out[12] = omega1 * x[12] + (omega * e12 - fx + z);
+    out[13] = omega1 * x[13] + (omega * e13 - fx);
+    out[14] = omega1 * x[14] + (omega * e14 - z);
+    out[15] = omega1 * x[15] + (omega * e15 - z - z);
+    out[16] = omega1 * x[16] + (omega * e16 - z);
+    out[17] = omega1 * x[17] + (omega * e17 - z + z);
+    out[18] = omega1 * x[18] + (omega * e18);
+}
+
+// outlet ghost values o[0..4] = q10..q14 from the momentum (not divided by rho) of x[19];
+// bcond.bc.F90:138-164
+template <typename R>
+EUP_HD void outlet_eq(const R *x, R *o)
+{
+    const R rf = R(3), qf = R(1.5), three = R(3), one = R(1);
+    R xj = +(x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]);
+    R yj = +(x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]);
+    R zj = +(x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]);
+    R cvsq = xj * xj + yj * yj + zj * zj;
+    R c10 = rf * (-xj - yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
+    R c11 = rf * (-xj - zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
+    R c12 = rf * (-xj + yj) + qf * (three * (xj - yj) * (xj - yj) - cvsq);
+    R c13 = rf * (-xj + zj) + qf * (three * (xj - zj) * (xj - zj) - cvsq);
+    R c14 = rf * (-xj) + qf * (three * (xj) * (xj)-cvsq);
+    o[0] = W<R>::p2 * (one + c10);
+    o[1] = W<R>::p2 * (one + c11);
+    o[2] = W<R>::p2 * (one + c12);
+    o[3] = W<R>::p2 * (one + c13);
+    o[4] = W<R>::p1 * (one + c14);
+}
+
+// inlet ghost values o[0..4] = q1..q5 at u = (u_in, 0, 0); bcond.bc.F90:300-314
+template <typename R>
+EUP_HD void inlet_eq(R u_in, R *o)
+{
+    const R rf = R(3), qf = R(1.5), three = R(3), one = R(1);
+    R xj = u_in, yj = R(0), zj = R(0);
+    R cvsq = xj * xj + yj * yj + zj * zj;
+    R c1 = rf * (xj - yj) + qf * (three * (xj - yj) * (xj - yj) - cvsq);
+    R c2 = rf * (xj - zj) + qf * (three * (xj - zj) * (xj - zj) - cvsq);
+    R c3 = rf * (xj + yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
+    R c4 = rf * (xj + zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
+    R c5 = rf * (xj) + qf * (three * (xj) * (xj)-cvsq);
+    o[0] = W<R>::p2 * (one + c1);
+    o[1] = W<R>::p2 * (one + c2);
+    o[2] = W<R>::p2 * (one + c3);
+    o[3] = W<R>::p2 * (one + c4);
+    o[4] = W<R>::p1 * (one + c5);
+}
+
+// ---------------------------------------------------------------------------------------------
+// storage traits: how a population is held in HBM and converted to the collision type
+// ---------------------------------------------------------------------------------------------
+struct bf16_t {
+    uint16_t bits;
+};
+
+template <typename S> struct Store;
+
+template <> struct Store<double> {
+    using compute = double;
+    static constexpr bool deviation = false;
+    static __device__ __forceinline__ double load(double v, int) { return v; }
+    static __device__ __forceinline__ double store(double f, int) { return f; }
+};
+
+template <> struct Store<float> {
+    using compute = float;
+    static constexpr bool deviation = false;
+    static __device__ __forceinline__ float load(float v, int) { return v; }
+    static __device__ __forceinline__ float store(float f, int) { return f; }
+};
+
+// deviation f - w_q in binary16; float -> half conversion rounds to nearest even
+template <> struct Store<_Float16> {
+    using compute = float;
+    static constexpr bool deviation = true;
+    static __device__ __forceinline__ float load(_Float16 v, int q) { return weight<float>(q) + (float)v; }
+    static __device__ __forceinline__ _Float16 store(float f, int q) { return (_Float16)(f - weight<float>(q)); }
+};
+
+// deviation f - w_q in bfloat16 with round-to-nearest-even
+template <> struct Store<bf16_t> {
+    using compute = float;
+    static constexpr bool deviation = true;
+    static __device__ __forceinline__ float to_float(bf16_t v) { return __uint_as_float((uint32_t)v.bits << 16); }
+    static __device__ __forceinline__ bf16_t from_float(float f)
+    {
+        uint32_t u = __float_as_uint(f);
+        uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+        bf16_t b;
+        b.bits = (uint16_t)(r >> 16);
+        return b;
+    }
+    static __device__ __forceinline__ float load(bf16_t v, int q) { return weight<float>(q) + to_float(v); }
+    static __device__ __forceinline__ bf16_t store(float f, int q) { return from_float(f - weight<float>(q)); }
+};
+
+} // namespace eup

@@ -1,0 +1,304 @@
+// Parity mode: the reference's serial FP64 time step on the GPU, bit for bit.
+//
+// One step of CODE/SRC/bgk3d.F90:128-138 under -DSERIAL -DORIGINAL -DOBSTACLES -DCYLINDER_INF is
+// bcond_obs -> bcond_bc -> movef -> col on ghosted arrays a01..a19.  Here it is four kernels on
+// two ghosted FP64 buffers A (current) and B (next):
+//   1. k_bb_gather / k_bb_scatter: the in-place bounce-back of bcond.obs.F90:47-77.  The serial
+//      loop reads a neighbour's opposite population, which is already overwritten when that
+//      neighbour is solid and comes earlier in (k, j, i) order; in that case the value read is
+//      the site's own old population (the neighbour wrote it from us).  Gathering every new value
+//      from old values first and scattering afterwards reproduces the serial result exactly,
+//      solid sites included, with no race.
+//   2. k_ghost: the ghost fills of bcond.bc.F90 for CYLINDER_INF (outflow + border fix :127-204,
+//      inflow :260-321, periodic z :385-409/:472-496, periodic y with the SERIAL corner copies
+//      :535-560/:600-625).  Every ghost cell is written with the value of its last writer in the
+//      reference's statement order; cells the reference never writes keep their value.  Ghosts
+//      are written into both A and B, because the next step's bounce-back reads the ghosts left
+//      by this step's fill (the one-step-stale seam of the serial path).
+//   3. k_stream_collide: movef.F90:43-68 fused with col.F90:96-235 (pull from A, collide, write
+//      the interior of B); the FP64 collision is the reference's expression order.
+#include "eup_internal.h"
+
+namespace eup {
+
+namespace {
+
+struct PArgs {
+    const double *A[19];
+    double *B[19];
+    int l, m, n;
+    double omega, fgrad, u_in;
+};
+
+__device__ __forceinline__ int64_t gidx(int lp, int mp, int i, int j, int k)
+{
+    return (int64_t)i + (int64_t)lp * ((int64_t)j + (int64_t)mp * (int64_t)k);
+}
+
+template <int Q> __device__ __forceinline__ constexpr bool precedes()
+{
+    // neighbour x + c_Q comes before x in the serial (k, j, i) loop order
+    return Dir<Q>::cz < 0 || (Dir<Q>::cz == 0 && (Dir<Q>::cy < 0 || (Dir<Q>::cy == 0 && Dir<Q>::cx < 0)));
+}
+
+template <int Q>
+__device__ __forceinline__ void bb_one(const PArgs &a, const uint8_t *solid, int i, int j, int k,
+                                       int64_t s, double *out)
+{
+    constexpr int cx = Dir<Q>::cx, cy = Dir<Q>::cy, cz = Dir<Q>::cz, op = Dir<Q>::opp;
+    const int lp = a.l + 2, mp = a.m + 2;
+    const int yi = i + cx, yj = j + cy, yk = k + cz;
+    bool ysolid = false;
+    if (yi >= 1 && yi <= a.l && yj >= 1 && yj <= a.m && yk >= 1 && yk <= a.n)
+        ysolid = solid[(int64_t)(yi - 1) + (int64_t)a.l * ((int64_t)(yj - 1) + (int64_t)a.m * (yk - 1))] != 0;
+    if (precedes<Q>() && ysolid)
+        out[Q] = a.A[Q][s];
+    else
+        out[Q] = a.A[op][gidx(lp, mp, yi, yj, yk)];
+}
+
+template <int... Qs> struct QSeq {};
+
+template <int... Qs>
+__device__ __forceinline__ void bb_all(QSeq<Qs...>, const PArgs &a, const uint8_t *solid, int i,
+                                       int j, int k, int64_t s, double *out)
+{
+    (bb_one<Qs>(a, solid, i, j, k, s, out), ...);
+}
+
+using Q18 = QSeq<0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17>;
+
+__global__ void k_bb_gather(PArgs a, const int32_t *list, int64_t nsolid, const uint8_t *solid,
+                            double *tmp)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsolid) return;
+    const int lp = a.l + 2, mp = a.m + 2;
+    const int64_t s = list[t];
+    const int i = (int)(s % lp);
+    const int j = (int)((s / lp) % mp);
+    const int k = (int)(s / ((int64_t)lp * mp));
+    double out[18];
+    bb_all(Q18{}, a, solid, i, j, k, s, out);
+#pragma unroll
+    for (int q = 0; q < 18; ++q) tmp[t * 18 + q] = out[q];
+}
+
+__global__ void k_bb_scatter(PArgs a, const int32_t *list, int64_t nsolid, const double *tmp)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsolid) return;
+    const int64_t s = list[t];
+#pragma unroll
+    for (int q = 0; q < 18; ++q) const_cast<double *>(a.A[q])[s] = tmp[t * 18 + q];
+}
+
+__device__ __forceinline__ void put2(const PArgs &a, int q, int64_t s, double v)
+{
+    const_cast<double *>(a.A[q])[s] = v;
+    a.B[q][s] = v;
+}
+
+// flattened work: [0, P) x = l+1 plane, [P, 2P) x = 0 plane, then l*m z-face columns, then
+// l*n y-face rows; P = (m+2)(n+2)
+__global__ void k_ghost(PArgs a)
+{
+    const int l = a.l, m = a.m, n = a.n, lp = l + 2, mp = m + 2;
+    const int64_t P = (int64_t)(m + 2) * (n + 2);
+    const int64_t Z = (int64_t)l * m, Y = (int64_t)l * n;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < P) { // outflow plane x = l+1
+        const int j = (int)(t % (m + 2)), k = (int)(t / (m + 2));
+        const bool jin = j >= 1 && j <= m, kin = k >= 1 && k <= n;
+        if (!jin && !kin) return; // corners of the plane: never written
+        const int sj = jin ? j : (j == 0 ? 1 : m);     // border fix sources (:171-204)
+        const int sk = kin ? k : (k == 0 ? 1 : n);
+        const int64_t src = gidx(lp, mp, l, sj, sk);
+        double x[19], o[5];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) x[q] = a.A[q][src];
+        outlet_eq<double>(x, o);
+        const int64_t d = gidx(lp, mp, l + 1, j, k);
+#pragma unroll
+        for (int u = 0; u < 5; ++u) put2(a, 9 + u, d, o[u]);
+        if (j == m + 1 && kin) put2(a, 9, d, a.A[9][gidx(lp, mp, 1, 1, k)]); // :612
+        return;
+    }
+    t -= P;
+    if (t < P) { // inflow plane x = 0
+        const int j = (int)(t % (m + 2)), k = (int)(t / (m + 2));
+        double o[5];
+        inlet_eq<double>(a.u_in, o);
+        const int64_t d = gidx(lp, mp, 0, j, k);
+#pragma unroll
+        for (int u = 0; u < 5; ++u) put2(a, u, d, o[u]);
+        if (j == 0 && k >= 1 && k <= n) put2(a, 2, d, a.A[2][gidx(lp, mp, l, m, k)]); // :547
+        return;
+    }
+    t -= P;
+    if (t < Z) { // z = n+1 and z = 0
+        const int i = 1 + (int)(t % l), j = 1 + (int)(t / l);
+        const int64_t up = gidx(lp, mp, i, j, n + 1), s1 = gidx(lp, mp, i, j, 1);
+        const int64_t dn = gidx(lp, mp, i, j, 0), sn = gidx(lp, mp, i, j, n);
+        const int qu[5] = {1, 8, 10, 14, 15}; // a02 a09 a11 a15 a16
+        const int qd[5] = {3, 5, 6, 12, 17};  // a04 a06 a07 a13 a18
+#pragma unroll
+        for (int u = 0; u < 5; ++u) put2(a, qu[u], up, a.A[qu[u]][s1]);
+#pragma unroll
+        for (int u = 0; u < 5; ++u) put2(a, qd[u], dn, a.A[qd[u]][sn]);
+        return;
+    }
+    t -= Z;
+    if (t < Y) { // y = 0 and y = m+1
+        const int i = 1 + (int)(t % l), k = 1 + (int)(t / l);
+        const int64_t lf = gidx(lp, mp, i, 0, k), sm = gidx(lp, mp, i, m, k);
+        const int64_t rt = gidx(lp, mp, i, m + 1, k), s1 = gidx(lp, mp, i, 1, k);
+        const int ql[5] = {2, 6, 7, 8, 11};   // a03 a07 a08 a09 a12
+        const int qr[5] = {0, 9, 15, 16, 17}; // a01 a10 a16 a17 a18
+#pragma unroll
+        for (int u = 0; u < 5; ++u) put2(a, ql[u], lf, a.A[ql[u]][sm]);
+#pragma unroll
+        for (int u = 0; u < 5; ++u) put2(a, qr[u], rt, a.A[qr[u]][s1]);
+    }
+}
+
+template <int Q>
+__device__ __forceinline__ void pull_one(const PArgs &a, int64_t s, int64_t lp, int64_t lpmp,
+                                         double *x)
+{
+    constexpr int cx = Dir<Q>::cx, cy = Dir<Q>::cy, cz = Dir<Q>::cz;
+    x[Q] = a.A[Q][s - cx - cy * lp - cz * lpmp];
+}
+
+template <int... Qs>
+__device__ __forceinline__ void pull_all(QSeq<Qs...>, const PArgs &a, int64_t s, int64_t lp,
+                                         int64_t lpmp, double *x)
+{
+    (pull_one<Qs>(a, s, lp, lpmp, x), ...);
+}
+
+__global__ __launch_bounds__(256) void k_stream_collide(PArgs a)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = (int64_t)a.l * a.m * a.n;
+    if (t >= N) return;
+    const int i = 1 + (int)(t % a.l);
+    const int64_t r = t / a.l;
+    const int j = 1 + (int)(r % a.m);
+    const int k = 1 + (int)(r / a.m);
+    const int64_t lp = a.l + 2, lpmp = lp * (a.m + 2);
+    const int64_t s = i + lp * j + lpmp * k;
+    double x[19];
+    pull_all(Q18{}, a, s, lp, lpmp, x);
+    x[18] = a.A[18][s];
+    collide<double>(x, x, a.omega, a.fgrad);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) a.B[q][s] = x[q];
+}
+
+PArgs make_args(Handle *h, int from)
+{
+    PArgs a;
+    double *src = static_cast<double *>(h->buf[from]);
+    double *dst = static_cast<double *>(h->buf[from ^ 1]);
+    for (int q = 0; q < 19; ++q) {
+        a.A[q] = src + (int64_t)q * h->stride;
+        a.B[q] = dst + (int64_t)q * h->stride;
+    }
+    a.l = h->g.l;
+    a.m = h->g.m;
+    a.n = h->g.n;
+    a.omega = h->cfg.omega;
+    a.fgrad = h->cfg.fgrad;
+    a.u_in = h->cfg.u_inflow;
+    return a;
+}
+
+inline unsigned nblocks(int64_t work, int bs) { return (unsigned)((work + bs - 1) / bs); }
+
+} // namespace
+
+int parity_prepare_mask(Handle *h)
+{
+    // solid list in the reference's bbox loop order (k, j, i), as ghosted offsets
+    const Geometry &g = h->g;
+    std::string dummy;
+    int32_t *obs_h = nullptr;
+    if (hip_check(h, hipHostMalloc((void **)&obs_h, sizeof(int32_t) * g.sites), "hipHostMalloc"))
+        return EUP_EHIP;
+    if (hip_check(h, hipMemcpy(obs_h, h->obs, sizeof(int32_t) * g.sites, hipMemcpyDeviceToHost),
+                  "hipMemcpy obs")) {
+        hipHostFree(obs_h);
+        return EUP_EHIP;
+    }
+    int64_t cnt = 0;
+    for (int64_t s = 0; s < g.sites; ++s) cnt += obs_h[s] == 1;
+    int32_t *list_h = nullptr;
+    if (cnt > 0 && hip_check(h, hipHostMalloc((void **)&list_h, sizeof(int32_t) * cnt), "hipHostMalloc")) {
+        hipHostFree(obs_h);
+        return EUP_EHIP;
+    }
+    int64_t c = 0;
+    for (int k = 1; k <= g.n; ++k)
+        for (int j = 1; j <= g.m; ++j)
+            for (int i = 1; i <= g.l; ++i) {
+                int64_t o = (int64_t)(i - 1) + (int64_t)g.l * ((int64_t)(j - 1) + (int64_t)g.m * (k - 1));
+                if (obs_h[o] == 1) list_h[c++] = (int32_t)ghosted_index(g, i, j, k);
+            }
+    hipHostFree(obs_h);
+    hipFree(h->solid_list);
+    hipFree(h->bb_tmp);
+    h->solid_list = nullptr;
+    h->bb_tmp = nullptr;
+    int rc = 0;
+    if (cnt > 0) {
+        if (hip_check(h, hipMalloc((void **)&h->solid_list, sizeof(int32_t) * cnt), "hipMalloc") ||
+            hip_check(h, hipMalloc((void **)&h->bb_tmp, sizeof(double) * 18 * cnt), "hipMalloc") ||
+            hip_check(h, hipMemcpy(h->solid_list, list_h, sizeof(int32_t) * cnt, hipMemcpyHostToDevice),
+                      "hipMemcpy list"))
+            rc = EUP_EHIP;
+    }
+    if (list_h) hipHostFree(list_h);
+    h->nsolid = cnt;
+    return rc;
+}
+
+int parity_step(Handle *h)
+{
+    const Geometry &g = h->g;
+    PArgs a = make_args(h, h->cur);
+    if (h->nsolid > 0) {
+        hipLaunchKernelGGL(k_bb_gather, dim3(nblocks(h->nsolid, 256)), dim3(256), 0, h->stream, a,
+                           h->solid_list, h->nsolid, h->solid, h->bb_tmp);
+        hipLaunchKernelGGL(k_bb_scatter, dim3(nblocks(h->nsolid, 256)), dim3(256), 0, h->stream, a,
+                           h->solid_list, h->nsolid, h->bb_tmp);
+    }
+    const int64_t gwork = 2 * (int64_t)(g.m + 2) * (g.n + 2) + (int64_t)g.l * g.m + (int64_t)g.l * g.n;
+    hipLaunchKernelGGL(k_ghost, dim3(nblocks(gwork, 256)), dim3(256), 0, h->stream, a);
+    hipLaunchKernelGGL(k_stream_collide, dim3(nblocks(g.sites, 256)), dim3(256), 0, h->stream, a);
+    if (hip_check(h, hipGetLastError(), "parity step launch")) return EUP_EHIP;
+    h->cur ^= 1;
+    return 0;
+}
+
+int parity_set_pop(Handle *h, int q)
+{
+    for (int b = 0; b < 2; ++b) {
+        double *dst = static_cast<double *>(h->buf[b]) + (int64_t)(q - 1) * h->stride;
+        if (hip_check(h, hipMemcpyAsync(dst, h->staging, sizeof(double) * h->g.ghosted,
+                                        hipMemcpyDeviceToDevice, h->stream), "hipMemcpyAsync"))
+            return EUP_EHIP;
+    }
+    return 0;
+}
+
+int parity_get_pop(Handle *h, int q)
+{
+    const double *src = static_cast<const double *>(h->buf[h->cur]) + (int64_t)(q - 1) * h->stride;
+    return hip_check(h, hipMemcpyAsync(h->staging, src, sizeof(double) * h->g.ghosted,
+                                       hipMemcpyDeviceToDevice, h->stream), "hipMemcpyAsync")
+               ? EUP_EHIP
+               : 0;
+}
+
+} // namespace eup

@@ -1,0 +1,473 @@
+// Physics mode: fused pull collide+stream, one launch per time step.
+//
+// The step has the arithmetic of the reference's col.F90:127-229 and the inlet/outlet formulas of
+// bcond.bc.F90, with the y/z periodicity applied by index wrap (ghosts always fresh, so there is
+// no stale seam and no frozen edge ghost), bounce-back taken from the site's own opposite
+// population (bcond.obs.F90:56-73 with fresh neighbours), and solid sites left untouched.
+//
+// Layout in HBM: populations are SoA, 19 arrays of l*m*n sites each (i fastest, no ghost layer),
+// two buffers (A current, B next).  The lattice is cut into tiles of 64 consecutive sites along x;
+// one wavefront updates one tile, so every load and store of a population is a contiguous 64-lane
+// access (256 B in FP32, 128 B in FP16).  Per tile a 4-byte code says whether the tile is pure
+// fluid (no solid site in or next to it: no mask traffic at all), all solid (skipped: no loads, no
+// stores), or mixed (the 64 per-site link masks are read: bit q set = the source of population q
+// is solid, bit 31 = the site itself is solid).  The codes are built once per mask with a
+// wavefront ballot (k_classify).
+//
+// Boundaries in x: the inlet ghost (x = 0) is the constant equilibrium at (u_in, 0, 0); the outlet
+// ghost plane (x = l+1, 5 populations) is written by the sites at x = l while they collide and
+// read one step later, so no extra pass over the lattice is needed.
+#include <type_traits>
+
+#include "eup_internal.h"
+
+namespace eup {
+
+namespace {
+
+template <typename S> struct XArgs {
+    using R = typename Store<S>::compute;
+    const S *__restrict__ A[19];
+    S *__restrict__ B[19];
+    const int32_t *__restrict__ tile_code;
+    const uint32_t *__restrict__ link;
+    const R *__restrict__ out_cur; // 5 x (m*n)
+    R *__restrict__ out_next;
+    R inl[5];
+    R omega, fgrad;
+    int l, m, n, tpr;
+    uint32_t tiles;
+    uint32_t chunk; // logical blocks per XCD
+};
+
+template <int... Qs> struct QSeq {};
+using Q18 = QSeq<0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17>;
+using Q19 = QSeq<0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18>;
+
+struct Rows {
+    // site offsets of the nine source rows, indexed [dy+1][dz+1]
+    uint32_t off[3][3];
+    int jj[3], kk[3];
+};
+
+template <typename S, bool XOPEN, bool MIXED, int Q>
+__device__ __forceinline__ void pull(const XArgs<S> &a, const Rows &r, int i, uint32_t s,
+                                     uint32_t lm, typename Store<S>::compute *x)
+{
+    using R = typename Store<S>::compute;
+    constexpr int cx = Dir<Q>::cx, cy = Dir<Q>::cy, cz = Dir<Q>::cz, op = Dir<Q>::opp;
+    // source site = x - c_Q
+    int si = i - cx;
+    if constexpr (!XOPEN) {
+        if constexpr (cx == 1) si = (i == 0) ? a.l - 1 : si;
+        if constexpr (cx == -1) si = (i == a.l - 1) ? 0 : si;
+    } else {
+        // the inlet / outlet lanes replace the value below; keep their load inside the row
+        if constexpr (cx == 1) si = (i == 0) ? 0 : si;
+        if constexpr (cx == -1) si = (i == a.l - 1) ? a.l - 1 : si;
+    }
+    const uint32_t src = r.off[1 - cy][1 - cz] + (uint32_t)si;
+    uint32_t addr_q = src;
+    const S *base = a.A[Q];
+    if constexpr (MIXED) {
+        if ((lm >> Q) & 1u) { // the source is solid: bounce back the site's own population
+            base = a.A[op];
+            addr_q = s;
+        }
+    }
+    R v = Store<S>::load(base[addr_q], Q);
+    if constexpr (XOPEN) {
+        if constexpr (cx == 1) {
+            if (i == 0) v = a.inl[Q];
+        }
+        if constexpr (cx == -1) {
+            if (i == a.l - 1) {
+                const int jj = r.jj[1 - cy], kk = r.kk[1 - cz];
+                v = a.out_cur[(Q - 9) * (size_t)(a.m * a.n) + jj + a.m * kk];
+            }
+        }
+    }
+    x[Q] = v;
+}
+
+template <typename S, bool XOPEN, bool MIXED, int... Qs>
+__device__ __forceinline__ void pull_all(QSeq<Qs...>, const XArgs<S> &a, const Rows &r, int i,
+                                         uint32_t s, uint32_t lm, typename Store<S>::compute *x)
+{
+    (pull<S, XOPEN, MIXED, Qs>(a, r, i, s, lm, x), ...);
+}
+
+template <typename S, int Q>
+__device__ __forceinline__ void put(const XArgs<S> &a, uint32_t s, const typename Store<S>::compute *x)
+{
+    a.B[Q][s] = Store<S>::store(x[Q], Q);
+}
+
+template <typename S, int... Qs>
+__device__ __forceinline__ void put_all(QSeq<Qs...>, const XArgs<S> &a, uint32_t s,
+                                        const typename Store<S>::compute *x)
+{
+    (put<S, Qs>(a, s, x), ...);
+}
+
+template <typename S, bool XOPEN, bool MIXED>
+__device__ __forceinline__ void update_site(const XArgs<S> &a, int i, int j, int k, uint32_t lm)
+{
+    using R = typename Store<S>::compute;
+    if constexpr (MIXED) {
+        if (lm & 0x80000000u) return; // solid site: frozen
+    }
+    const int m = a.m, n = a.n, l = a.l;
+    Rows r;
+    r.jj[0] = (j == 0) ? m - 1 : j - 1; // source row j - 1   (dy = -1)
+    r.jj[1] = j;
+    r.jj[2] = (j == m - 1) ? 0 : j + 1;
+    r.kk[0] = (k == 0) ? n - 1 : k - 1;
+    r.kk[1] = k;
+    r.kk[2] = (k == n - 1) ? 0 : k + 1;
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+            r.off[dy][dz] = (uint32_t)l * (uint32_t)(r.jj[dy] + m * r.kk[dz]);
+    const uint32_t s = r.off[1][1] + (uint32_t)i;
+    R x[19];
+    pull_all<S, XOPEN, MIXED>(Q18{}, a, r, i, s, lm, x);
+    x[18] = Store<S>::load(a.A[18][s], 18);
+    collide<R>(x, x, a.omega, a.fgrad);
+    put_all<S>(Q19{}, a, s, x);
+    if constexpr (XOPEN) {
+        if (i == l - 1) {
+            R o[5];
+            outlet_eq<R>(x, o);
+            const size_t mn = (size_t)m * n;
+#pragma unroll
+            for (int u = 0; u < 5; ++u) a.out_next[u * mn + j + (size_t)m * k] = o[u];
+        }
+    }
+}
+
+template <typename S, bool XOPEN>
+__global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
+{
+    // XCD-aware block order: hardware block b runs on XCD b % 8; give each XCD a contiguous
+    // range of tiles so that the lines shared by neighbouring tiles stay in one L2.
+    const uint32_t b = blockIdx.x;
+    const uint32_t lb = (b & 7u) * a.chunk + (b >> 3);
+    const uint32_t t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kTile) + (threadIdx.x >> 6));
+    if (t >= a.tiles) return;
+    const int32_t code = __builtin_amdgcn_readfirstlane(a.tile_code[t]);
+    if (code == kTileSolid) return;
+    const uint32_t row = t / (uint32_t)a.tpr;
+    const int i = (int)(t - row * (uint32_t)a.tpr) * kTile + (int)(threadIdx.x & 63);
+    if (i >= a.l) return;
+    const int j = (int)(row % (uint32_t)a.m);
+    const int k = (int)(row / (uint32_t)a.m);
+    if (code == kTileFluid) {
+        update_site<S, XOPEN, false>(a, i, j, k, 0u);
+    } else {
+        const uint32_t lm = a.link[(size_t)code * kTile + (threadIdx.x & 63)];
+        update_site<S, XOPEN, true>(a, i, j, k, lm);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// mask preparation
+// ---------------------------------------------------------------------------------------------
+
+// per site: link mask (bit q = source x - c_q solid, bit 31 = self solid); per tile: class via
+// ballot.  cls: 0 fluid, 1 mixed, 2 all solid
+__global__ __launch_bounds__(kBlock) void k_classify(const uint8_t *__restrict__ solid, int l, int m,
+                                                     int n, int tpr, uint32_t tiles, int xopen,
+                                                     int noskip, uint8_t *__restrict__ cls,
+                                                     uint32_t *__restrict__ link_all)
+{
+    const uint32_t t = blockIdx.x * (kBlock / kTile) + (threadIdx.x >> 6);
+    if (t >= tiles) return; // whole wavefront leaves together
+    const uint32_t row = t / (uint32_t)tpr;
+    const int i = (int)(t - row * (uint32_t)tpr) * kTile + (int)(threadIdx.x & 63);
+    const int j = (int)(row % (uint32_t)m), k = (int)(row / (uint32_t)m);
+    const bool active = i < l;
+    uint32_t lm = 0;
+    if (active) {
+        auto S = [&](int ii, int jj, int kk) -> bool {
+            if (ii < 0 || ii >= l) {
+                if (xopen) return false;
+                ii = (ii + l) % l;
+            }
+            jj = (jj + m) % m;
+            kk = (kk + n) % n;
+            return solid[(size_t)ii + (size_t)l * ((size_t)jj + (size_t)m * kk)] != 0;
+        };
+        const int cxs[18] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0};
+        const int cys[18] = {-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1};
+        const int czs[18] = {0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1};
+#pragma unroll
+        for (int q = 0; q < 18; ++q)
+            if (S(i - cxs[q], j - cys[q], k - czs[q])) lm |= 1u << q;
+        if (S(i, j, k)) lm |= 0x80000000u;
+    }
+    const uint64_t any_link = __ballot(active && lm != 0);
+    const uint64_t not_solid = __ballot(active && !(lm & 0x80000000u));
+    uint8_t c;
+    if (noskip) c = 1;
+    else if (not_solid == 0) c = 2;
+    else if (any_link == 0) c = 0;
+    else c = 1;
+    if ((threadIdx.x & 63) == 0) cls[t] = c;
+    link_all[(size_t)t * kTile + (threadIdx.x & 63)] = lm;
+}
+
+__global__ void k_compact_links(const int32_t *__restrict__ code, const uint32_t *__restrict__ link_all,
+                                uint32_t tiles, uint32_t *__restrict__ link)
+{
+    const uint32_t t = blockIdx.x * (kBlock / kTile) + (threadIdx.x >> 6);
+    if (t >= tiles) return;
+    const int32_t c = code[t];
+    if (c < 0) return;
+    link[(size_t)c * kTile + (threadIdx.x & 63)] = link_all[(size_t)t * kTile + (threadIdx.x & 63)];
+}
+
+// ---------------------------------------------------------------------------------------------
+// state conversion and outlet refresh
+// ---------------------------------------------------------------------------------------------
+template <typename S>
+__global__ void k_from_staging(const double *__restrict__ stg, S *__restrict__ d0, S *__restrict__ d1,
+                               int l, int m, int n, int q)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = (int64_t)l * m * n;
+    if (t >= N) return;
+    const int i = (int)(t % l);
+    const int64_t r = t / l;
+    const int j = (int)(r % m), k = (int)(r / m);
+    const double v = stg[(int64_t)(i + 1) + (int64_t)(l + 2) * ((int64_t)(j + 1) + (int64_t)(m + 2) * (k + 1))];
+    using R = typename Store<S>::compute;
+    const S sv = Store<S>::store((R)v, q);
+    d0[t] = sv;
+    d1[t] = sv;
+}
+
+template <typename S>
+__global__ void k_to_staging(const S *__restrict__ src, double *__restrict__ stg, int l, int m, int n,
+                             int q)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = (int64_t)l * m * n;
+    if (t >= N) return;
+    const int i = (int)(t % l);
+    const int64_t r = t / l;
+    const int j = (int)(r % m), k = (int)(r / m);
+    stg[(int64_t)(i + 1) + (int64_t)(l + 2) * ((int64_t)(j + 1) + (int64_t)(m + 2) * (k + 1))] =
+        (double)Store<S>::load(src[t], q);
+}
+
+template <typename S>
+__global__ void k_outlet_refresh(XArgs<S> a, typename Store<S>::compute *o0,
+                                 typename Store<S>::compute *o1)
+{
+    using R = typename Store<S>::compute;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t mn = (int64_t)a.m * a.n;
+    if (t >= mn) return;
+    const uint32_t s = (uint32_t)(a.l - 1) + (uint32_t)a.l * (uint32_t)t;
+    R x[19], o[5];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) x[q] = Store<S>::load(a.A[q][s], q);
+    outlet_eq<R>(x, o);
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+        o0[u * mn + t] = o[u];
+        o1[u * mn + t] = o[u];
+    }
+}
+
+template <typename S> XArgs<S> make_args(Handle *h)
+{
+    using R = typename Store<S>::compute;
+    XArgs<S> a;
+    S *src = static_cast<S *>(h->buf[h->cur]);
+    S *dst = static_cast<S *>(h->buf[h->cur ^ 1]);
+    for (int q = 0; q < 19; ++q) {
+        a.A[q] = src + (int64_t)q * h->stride;
+        a.B[q] = dst + (int64_t)q * h->stride;
+    }
+    a.tile_code = h->tile_code;
+    a.link = h->link_mask;
+    a.out_cur = static_cast<const R *>(h->outlet[h->cur]);
+    a.out_next = static_cast<R *>(h->outlet[h->cur ^ 1]);
+    R inl[5];
+    inlet_eq<R>((R)h->cfg.u_inflow, inl);
+    for (int u = 0; u < 5; ++u) a.inl[u] = inl[u];
+    a.omega = (R)h->cfg.omega;
+    a.fgrad = (R)h->cfg.fgrad;
+    a.l = h->g.l;
+    a.m = h->g.m;
+    a.n = h->g.n;
+    a.tpr = h->g.tiles_per_row;
+    a.tiles = (uint32_t)h->g.tiles;
+    const uint32_t nb = (uint32_t)((h->g.tiles + 3) / 4);
+    a.chunk = (nb + 7) / 8;
+    return a;
+}
+
+inline bool x_open(const Handle *h) { return h->cfg.front != EUP_FACE_PERIODIC; }
+
+template <typename S> int step_t(Handle *h)
+{
+    XArgs<S> a = make_args<S>(h);
+    const unsigned grid = 8u * a.chunk; // multiple of 8 so the XCD remap is a bijection
+    if (x_open(h))
+        hipLaunchKernelGGL((k_fused<S, true>), dim3(grid), dim3(kBlock), 0, h->stream, a);
+    else
+        hipLaunchKernelGGL((k_fused<S, false>), dim3(grid), dim3(kBlock), 0, h->stream, a);
+    if (hip_check(h, hipGetLastError(), "fused step launch")) return EUP_EHIP;
+    h->cur ^= 1;
+    return 0;
+}
+
+template <typename S> int refresh_t(Handle *h)
+{
+    using R = typename Store<S>::compute;
+    XArgs<S> a = make_args<S>(h);
+    const int64_t mn = (int64_t)h->g.m * h->g.n;
+    hipLaunchKernelGGL((k_outlet_refresh<S>), dim3((unsigned)((mn + 255) / 256)), dim3(256), 0,
+                       h->stream, a, static_cast<R *>(h->outlet[0]), static_cast<R *>(h->outlet[1]));
+    return hip_check(h, hipGetLastError(), "outlet refresh") ? EUP_EHIP : 0;
+}
+
+template <typename S> int set_t(Handle *h, int q)
+{
+    S *d0 = static_cast<S *>(h->buf[0]) + (int64_t)(q - 1) * h->stride;
+    S *d1 = static_cast<S *>(h->buf[1]) + (int64_t)(q - 1) * h->stride;
+    hipLaunchKernelGGL((k_from_staging<S>), dim3((unsigned)((h->g.sites + 255) / 256)), dim3(256), 0,
+                       h->stream, h->staging, d0, d1, h->g.l, h->g.m, h->g.n, q - 1);
+    return hip_check(h, hipGetLastError(), "set pops") ? EUP_EHIP : 0;
+}
+
+template <typename S> int get_t(Handle *h, int q)
+{
+    const S *s = static_cast<const S *>(h->buf[h->cur]) + (int64_t)(q - 1) * h->stride;
+    if (hip_check(h, hipMemsetAsync(h->staging, 0, sizeof(double) * h->g.ghosted, h->stream), "memset"))
+        return EUP_EHIP;
+    hipLaunchKernelGGL((k_to_staging<S>), dim3((unsigned)((h->g.sites + 255) / 256)), dim3(256), 0,
+                       h->stream, s, h->staging, h->g.l, h->g.m, h->g.n, q - 1);
+    return hip_check(h, hipGetLastError(), "get pops") ? EUP_EHIP : 0;
+}
+
+template <typename F> int dispatch(Handle *h, F &&f)
+{
+    switch (h->cfg.storage) {
+    case EUP_STORE_F64: return f((double *)nullptr);
+    case EUP_STORE_F32: return f((float *)nullptr);
+    case EUP_STORE_F16: return f((_Float16 *)nullptr);
+    case EUP_STORE_BF16: return f((bf16_t *)nullptr);
+    }
+    return fail(h, EUP_EINVAL, "unknown storage");
+}
+
+} // namespace
+
+int physics_step(Handle *h)
+{
+    if (h->outlet_dirty) {
+        int rc = physics_refresh_outlet(h);
+        if (rc) return rc;
+    }
+    return dispatch(h, [&](auto *p) { return step_t<std::remove_pointer_t<decltype(p)>>(h); });
+}
+
+int physics_refresh_outlet(Handle *h)
+{
+    h->outlet_dirty = false;
+    return dispatch(h, [&](auto *p) { return refresh_t<std::remove_pointer_t<decltype(p)>>(h); });
+}
+
+int physics_set_pop(Handle *h, int q)
+{
+    h->outlet_dirty = true;
+    return dispatch(h, [&](auto *p) { return set_t<std::remove_pointer_t<decltype(p)>>(h, q); });
+}
+
+int physics_get_pop(Handle *h, int q)
+{
+    return dispatch(h, [&](auto *p) { return get_t<std::remove_pointer_t<decltype(p)>>(h, q); });
+}
+
+int physics_prepare_mask(Handle *h)
+{
+    const Geometry &g = h->g;
+    const bool noskip = (h->cfg.flags & EUP_FLAG_NO_TILE_SKIP) != 0;
+    uint8_t *cls = nullptr;
+    uint32_t *link_all = nullptr;
+    int rc = 0;
+    if (hip_check(h, hipMalloc((void **)&cls, g.tiles), "hipMalloc") ||
+        hip_check(h, hipMalloc((void **)&link_all, sizeof(uint32_t) * kTile * g.tiles), "hipMalloc")) {
+        hipFree(cls);
+        return EUP_EHIP;
+    }
+    const unsigned grid = (unsigned)((g.tiles + 3) / 4);
+    hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kBlock), 0, h->stream, h->solid, g.l, g.m, g.n,
+                       g.tiles_per_row, (uint32_t)g.tiles, (int)x_open(h), (int)noskip, cls, link_all);
+    if (hip_check(h, hipGetLastError(), "classify")) rc = EUP_EHIP;
+    uint8_t *cls_h = nullptr;
+    int32_t *code_h = nullptr;
+    if (!rc && (hip_check(h, hipHostMalloc((void **)&cls_h, g.tiles), "hipHostMalloc") ||
+                hip_check(h, hipHostMalloc((void **)&code_h, sizeof(int32_t) * g.tiles), "hipHostMalloc")))
+        rc = EUP_EHIP;
+    if (!rc && hip_check(h, hipMemcpyAsync(cls_h, cls, g.tiles, hipMemcpyDeviceToHost, h->stream), "copy") )
+        rc = EUP_EHIP;
+    if (!rc && hip_check(h, hipStreamSynchronize(h->stream), "sync")) rc = EUP_EHIP;
+    int64_t nf = 0, nm = 0, ns = 0;
+    if (!rc) {
+        for (int64_t t = 0; t < g.tiles; ++t) {
+            if (cls_h[t] == 0) { code_h[t] = kTileFluid; ++nf; }
+            else if (cls_h[t] == 2) { code_h[t] = kTileSolid; ++ns; }
+            else code_h[t] = (int32_t)(nm++);
+        }
+        h->tiles_fluid = nf;
+        h->tiles_mixed = nm;
+        h->tiles_solid = ns;
+        hipFree(h->tile_code);
+        hipFree(h->link_mask);
+        h->tile_code = nullptr;
+        h->link_mask = nullptr;
+        if (hip_check(h, hipMalloc((void **)&h->tile_code, sizeof(int32_t) * g.tiles), "hipMalloc") ||
+            hip_check(h, hipMalloc((void **)&h->link_mask, sizeof(uint32_t) * kTile * (nm > 0 ? nm : 1)),
+                      "hipMalloc") ||
+            hip_check(h, hipMemcpyAsync(h->tile_code, code_h, sizeof(int32_t) * g.tiles,
+                                        hipMemcpyHostToDevice, h->stream), "copy"))
+            rc = EUP_EHIP;
+    }
+    if (!rc && nm > 0) {
+        hipLaunchKernelGGL(k_compact_links, dim3(grid), dim3(kBlock), 0, h->stream, h->tile_code,
+                           link_all, (uint32_t)g.tiles, h->link_mask);
+        if (hip_check(h, hipGetLastError(), "compact links")) rc = EUP_EHIP;
+    }
+    if (!rc && nm > 0) {
+        // mixed tile list for the momentum-exchange reduction
+        int32_t *list_h = nullptr;
+        if (hip_check(h, hipHostMalloc((void **)&list_h, sizeof(int32_t) * nm), "hipHostMalloc")) rc = EUP_EHIP;
+        if (!rc) {
+            for (int64_t t = 0; t < g.tiles; ++t)
+                if (code_h[t] >= 0) list_h[code_h[t]] = (int32_t)t;
+            hipFree(h->mixed_list);
+            h->mixed_list = nullptr;
+            if (hip_check(h, hipMalloc((void **)&h->mixed_list, sizeof(int32_t) * nm), "hipMalloc") ||
+                hip_check(h, hipMemcpyAsync(h->mixed_list, list_h, sizeof(int32_t) * nm,
+                                            hipMemcpyHostToDevice, h->stream), "copy"))
+                rc = EUP_EHIP;
+            if (!rc && hip_check(h, hipStreamSynchronize(h->stream), "sync")) rc = EUP_EHIP;
+            hipHostFree(list_h);
+        }
+    }
+    if (!rc && hip_check(h, hipStreamSynchronize(h->stream), "sync")) rc = EUP_EHIP;
+    hipFree(cls);
+    hipFree(link_all);
+    if (cls_h) hipHostFree(cls_h);
+    if (code_h) hipHostFree(code_h);
+    h->outlet_dirty = true;
+    return rc;
+}
+
+} // namespace eup

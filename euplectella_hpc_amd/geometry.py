@@ -164,24 +164,29 @@ def grey_spheres(l, m, n, n_spheres=12, rmin=4.0, rmax=8.0, sigma=1.5, seed=0, r
     return np.clip(s, 0.0, 1.0)
 
 
-def initial_state(l, m, n, u_in, noise=1e-4, seed=0):
-    """Ghosted initial populations ``[19, n+2, m+2, l+2]`` (float64).
+def initial_pops(l, m, n, u_in, noise=1e-4, seed=0):
+    """Yield (q, ghosted population q - 1 as ``[n+2, m+2, l+2]`` float64) for q = 1..19.
 
     rho = 1 and u = (u_in, U[-noise, noise], 0) at every site including the ghost layer, each
-    site set to its own equilibrium, written in the reference's equilibrium arithmetic
-    (CODE/SRC/init.F90:201-239 form: ``p*(1 + rf*cu + qf*(3*cu*cu - u^2))``).
+    site set to its own equilibrium ``w*(1 + (3 cu + 1.5 (3 cu^2 - u^2)))``.  One population
+    at a time keeps the host footprint at a few arrays for large lattices.
     """
     rng = np.random.default_rng(seed)
     shape = (n + 2, m + 2, l + 2)
-    ux = np.full(shape, float(u_in))
+    ux = float(u_in)
     uy = rng.uniform(-noise, noise, size=shape)
-    uz = np.zeros(shape)
-    usq = ux * ux + uy * uy + uz * uz
-    pops = np.empty((19,) + shape, dtype=np.float64)
+    usq = ux * ux + uy * uy
     for q in range(19):
         cx, cy, cz = (float(c) for c in d3q19.C[q])
-        cu = cx * ux + cy * uy + cz * uz
-        pops[q] = d3q19.W[q] * (1.0 + (3.0 * cu + 1.5 * (3.0 * cu * cu - usq)))
+        cu = cx * ux + cy * uy
+        yield q + 1, d3q19.W[q] * (1.0 + (3.0 * cu + 1.5 * (3.0 * cu * cu - usq)))
+
+
+def initial_state(l, m, n, u_in, noise=1e-4, seed=0):
+    """Ghosted initial populations ``[19, n+2, m+2, l+2]`` (float64); see ``initial_pops``."""
+    pops = np.empty((19, n + 2, m + 2, l + 2), dtype=np.float64)
+    for q, arr in initial_pops(l, m, n, u_in, noise, seed):
+        pops[q - 1] = arr
     return pops
 
 

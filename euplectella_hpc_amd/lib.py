@@ -1,0 +1,251 @@
+"""ctypes binding of libeup.so (the C-ABI in include/eup.h) and a thin Python host.
+
+``Lattice`` mirrors the reference's per-run flow: build the boundary flags and collision
+parameters (build_bcond + hencol), read the obstacle mask (build_obs), restore the populations
+(restore_raw), then call the time step (boundaries / propagation / collision) and the diagnostics
+(drag, probe_global, diagno).  Every compute call goes to the HIP library; there is no CPU
+fallback: if ``libeup.so`` is missing the import of this module's functions raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libeup.so")
+
+EUP_OK = 0
+ERRORS = {-1: "EUP_EINVAL", -2: "EUP_EHIP", -3: "EUP_ESTATE", -4: "EUP_ERANGE"}
+
+STORE_F64, STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2, 3
+STORAGE = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
+MODE_PARITY, MODE_PHYSICS = 0, 1
+FACE_PERIODIC, FACE_INFLOW, FACE_OUTFLOW = 0, 4, 5
+FLAG_NO_TILE_SKIP = 1
+
+#: every function the header declares (checked by the CPU tests)
+EXPORTS = ("eup_create", "eup_destroy", "eup_last_error", "eup_set_mask", "eup_set_pops",
+           "eup_get_pops", "eup_set_state", "eup_get_state", "eup_step", "eup_step_timed",
+           "eup_sync", "eup_drag_cv", "eup_drag_mem", "eup_probe", "eup_diagno", "eup_get_info",
+           "eup_stream", "eup_abi_version")
+
+
+class EupConfig(ctypes.Structure):
+    _fields_ = [("l", ctypes.c_int32), ("m", ctypes.c_int32), ("n", ctypes.c_int32),
+                ("mode", ctypes.c_int32), ("storage", ctypes.c_int32),
+                ("front", ctypes.c_int32), ("rear", ctypes.c_int32),
+                ("up", ctypes.c_int32), ("down", ctypes.c_int32),
+                ("left", ctypes.c_int32), ("right", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("flags", ctypes.c_uint32),
+                ("omega", ctypes.c_double), ("fgrad", ctypes.c_double),
+                ("u_inflow", ctypes.c_double)]
+
+
+class EupInfo(ctypes.Structure):
+    _fields_ = [("sites", ctypes.c_int64), ("nsolid", ctypes.c_int64),
+                ("tiles", ctypes.c_int64), ("tiles_fluid", ctypes.c_int64),
+                ("tiles_mixed", ctypes.c_int64), ("tiles_solid", ctypes.c_int64),
+                ("bbox", ctypes.c_int32 * 6), ("bytes_per_site", ctypes.c_int32),
+                ("device_bytes", ctypes.c_int64)]
+
+
+class EupError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libeup.so (raises FileNotFoundError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
+    i32 = ctypes.c_int32
+    L.eup_create.argtypes = [ctypes.POINTER(EupConfig), ctypes.POINTER(vp)]
+    L.eup_destroy.argtypes = [vp]
+    L.eup_destroy.restype = None
+    L.eup_last_error.argtypes = [vp]
+    L.eup_last_error.restype = ctypes.c_char_p
+    L.eup_set_mask.argtypes = [vp, ip]
+    L.eup_set_pops.argtypes = [vp, i32, dp]
+    L.eup_get_pops.argtypes = [vp, i32, dp]
+    L.eup_set_state.argtypes = [vp, dp]
+    L.eup_get_state.argtypes = [vp, dp]
+    L.eup_step.argtypes = [vp, i32]
+    L.eup_step_timed.argtypes = [vp, i32, dp]
+    L.eup_sync.argtypes = [vp]
+    L.eup_drag_cv.argtypes = [vp, ip, dp, dp]
+    L.eup_drag_mem.argtypes = [vp, dp]
+    L.eup_probe.argtypes = [vp, i32, i32, i32, dp]
+    L.eup_diagno.argtypes = [vp, dp]
+    L.eup_get_info.argtypes = [vp, ctypes.POINTER(EupInfo)]
+    L.eup_stream.argtypes = [vp]
+    L.eup_stream.restype = vp
+    L.eup_abi_version.argtypes = []
+    L.eup_abi_version.restype = i32
+    _lib = L
+    return L
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def omega_from_svisc(svisc):
+    """hencol.F90:30: omega = 2/(6*svisc + 1), in double."""
+    return 2.0 / (6.0 * float(svisc) + 1.0)
+
+
+def svisc_from_tau(tau):
+    """Kinematic viscosity (2 tau - 1)/6 as a deck would carry it."""
+    return (2.0 * float(tau) - 1.0) / 6.0
+
+
+class Lattice:
+    """One D3Q19 lattice on one GPU (a libeup handle).
+
+    ``mode`` is "parity" (the reference's serial FP64 step, bit for bit) or "physics" (fused
+    pull kernel, ``storage`` f64 / f32 / f16 / bf16).  ``x_open`` selects inlet/outlet x faces
+    (the CYLINDER_INF build) or a periodic x axis.
+    """
+
+    def __init__(self, l, m, n, *, svisc, u_inflow, mode="physics", storage="f32", x_open=True,
+                 fgrad=0.0, device=0, tile_skip=True):
+        self.L = load()
+        self.shape = (l, m, n)
+        self.svisc = float(svisc)
+        self.omega = omega_from_svisc(svisc)
+        self.u_inflow = float(u_inflow)
+        cfg = EupConfig()
+        cfg.l, cfg.m, cfg.n = l, m, n
+        cfg.mode = MODE_PARITY if mode == "parity" else MODE_PHYSICS
+        cfg.storage = STORAGE[storage]
+        cfg.front = FACE_OUTFLOW if x_open else FACE_PERIODIC
+        cfg.rear = FACE_INFLOW if x_open else FACE_PERIODIC
+        cfg.up = cfg.down = cfg.left = cfg.right = FACE_PERIODIC
+        cfg.device = device
+        cfg.flags = 0 if tile_skip else FLAG_NO_TILE_SKIP
+        cfg.omega = self.omega
+        cfg.fgrad = float(fgrad)
+        cfg.u_inflow = self.u_inflow
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        rc = self.L.eup_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != EUP_OK:
+            raise EupError(rc, self.L.eup_last_error(None).decode())
+        self.h = h
+
+    # -- plumbing -------------------------------------------------------------------------------
+    def _check(self, rc):
+        if rc != EUP_OK:
+            raise EupError(rc, self.L.eup_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.eup_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state ----------------------------------------------------------------------------------
+    def set_mask(self, mask):
+        """mask[k, j, i] (n, m, l), nonzero = solid."""
+        l, m, n = self.shape
+        obs = np.ascontiguousarray(np.asarray(mask) != 0, dtype=np.int32)
+        if obs.shape != (n, m, l):
+            raise ValueError(f"mask shape {obs.shape} != {(n, m, l)}")
+        self._check(self.L.eup_set_mask(self.h, obs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+
+    def set_state(self, pops):
+        l, m, n = self.shape
+        p = np.ascontiguousarray(pops, dtype=np.float64)
+        if p.shape != (19, n + 2, m + 2, l + 2):
+            raise ValueError("pops must have shape (19, n+2, m+2, l+2)")
+        self._check(self.L.eup_set_state(self.h, _dptr(p)))
+
+    def set_pops(self, q, arr):
+        """Population q (1..19) as a ghosted ``[n+2, m+2, l+2]`` float64 array."""
+        l, m, n = self.shape
+        p = np.ascontiguousarray(arr, dtype=np.float64)
+        if p.shape != (n + 2, m + 2, l + 2):
+            raise ValueError("population must have shape (n+2, m+2, l+2)")
+        self._check(self.L.eup_set_pops(self.h, int(q), _dptr(p)))
+
+    def get_pops(self, q):
+        l, m, n = self.shape
+        out = np.empty((n + 2, m + 2, l + 2), dtype=np.float64)
+        self._check(self.L.eup_get_pops(self.h, int(q), _dptr(out)))
+        return out
+
+    def get_state(self):
+        l, m, n = self.shape
+        out = np.empty((19, n + 2, m + 2, l + 2), dtype=np.float64)
+        self._check(self.L.eup_get_state(self.h, _dptr(out)))
+        return out
+
+    # -- stepping -------------------------------------------------------------------------------
+    def step(self, nsteps=1):
+        self._check(self.L.eup_step(self.h, int(nsteps)))
+
+    def step_timed(self, nsteps):
+        ms = ctypes.c_double()
+        self._check(self.L.eup_step_timed(self.h, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        self._check(self.L.eup_sync(self.h))
+
+    # -- diagnostics ----------------------------------------------------------------------------
+    def drag_cv(self, box):
+        b = np.ascontiguousarray(box, dtype=np.int32)
+        fx, fy = ctypes.c_double(), ctypes.c_double()
+        self._check(self.L.eup_drag_cv(self.h, b.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                       ctypes.byref(fx), ctypes.byref(fy)))
+        return fx.value, fy.value
+
+    def drag_mem(self):
+        F = np.zeros(3)
+        self._check(self.L.eup_drag_mem(self.h, _dptr(F)))
+        return F
+
+    def probe(self, i, j, k):
+        out = np.zeros(4)
+        self._check(self.L.eup_probe(self.h, int(i), int(j), int(k), _dptr(out)))
+        return out
+
+    def diagno(self):
+        out = np.zeros(5)
+        self._check(self.L.eup_diagno(self.h, _dptr(out)))
+        return out
+
+    def info(self):
+        inf = EupInfo()
+        self._check(self.L.eup_get_info(self.h, ctypes.byref(inf)))
+        return {k: (list(getattr(inf, k)) if k == "bbox" else getattr(inf, k))
+                for k, _ in EupInfo._fields_}
+
+    def stream(self):
+        return self.L.eup_stream(self.h)
+
+
+def drag_boxes(lx, ly, lz, radius=100, deltas=(52, 57, 62, 67)):
+    """Control boxes of the reference drag diagnostic (CODE/SRC/drag.F90:48-58)."""
+    return [(d, (lx // 4 - radius - d, lx // 4 + radius + d, ly // 2 - radius - d,
+                 ly // 2 + radius + d, 1, lz)) for d in deltas]

@@ -1,0 +1,162 @@
+/*
+ * eup.h -- C-ABI of the MI355X D3Q19 BGK hot path (libeup.so).
+ *
+ * Drop-in for the per-time-step routines of the sponge-flow solver's serial/OpenMP path.  In the
+ * reference these are argument-less Fortran subroutines sharing module state (`use storage`,
+ * CODE/SRC/storage.F90:26-437); here the state lives on the GPU behind an opaque handle and every
+ * call takes plain pointers and sizes, so a Fortran host binds it with ISO_C_BINDING (see
+ * euplectella_hpc_amd/fortran/eup_mod.F90) and Python with ctypes.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative EUP_E* code; the message is available
+ *     from eup_last_error(handle) (or eup_last_error(NULL) for failures without a handle).  This
+ *     replaces the reference's `write(6,*) "ERROR..."; stop`.
+ *   - One handle = one GPU + one HIP stream; calls on a handle are synchronous with respect to the
+ *     host arrays they take (the host may reuse a buffer when a call returns).  A handle is not
+ *     thread-safe; use one handle per host thread.
+ *   - Host arrays are owned by the caller and never retained.  Device memory is owned by the
+ *     handle and released by eup_destroy.
+ *   - Lattice indices are 1-based like the reference: interior sites i = 1..l, j = 1..m,
+ *     k = 1..n; ghost layers 0 and l+1 (m+1, n+1).  Ghosted host arrays have
+ *     (l+2)(m+2)(n+2) elements with i fastest (Fortran order of a01..a19, alloca.F90:94-133).
+ *   - Population q is 1..19 with the reference's direction table (bgk3d.F90:31-49).
+ */
+#ifndef EUP_H
+#define EUP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EUP_ABI_VERSION 1
+
+/* error codes */
+#define EUP_OK 0
+#define EUP_EINVAL (-1)   /* invalid argument or configuration */
+#define EUP_EHIP (-2)     /* HIP runtime error (no device, out of memory, launch failure) */
+#define EUP_ESTATE (-3)   /* call order violated (e.g. step before mask and populations are set) */
+#define EUP_ERANGE (-4)   /* index outside the lattice */
+
+/* population storage on the device */
+#define EUP_STORE_F64 0   /* full f in FP64 (parity mode requires this) */
+#define EUP_STORE_F32 1   /* full f in FP32, FP32 collision */
+#define EUP_STORE_F16 2   /* deviation f_q - w_q in IEEE binary16, FP32 collision */
+#define EUP_STORE_BF16 3  /* deviation f_q - w_q in bfloat16, FP32 collision */
+
+/* time-step semantics */
+#define EUP_MODE_PARITY 0  /* the reference's serial step, bit for bit (FP64, ghosted arrays) */
+#define EUP_MODE_PHYSICS 1 /* fused pull collide+stream: y/z periodic by index wrap, bounce-back
+                              from the site's own opposite population, solid sites frozen */
+
+/* face codes of build.bcond.F90 (0 periodic, 4 inflow, 5 outflow) */
+#define EUP_FACE_PERIODIC 0
+#define EUP_FACE_INFLOW 4
+#define EUP_FACE_OUTFLOW 5
+
+/* flags */
+#define EUP_FLAG_NO_TILE_SKIP 1u   /* physics mode: process every tile (disables the all-solid
+                                      tile skip and the pure-fluid fast path) */
+
+typedef struct eup_handle eup_handle;
+
+typedef struct eup_config {
+    int32_t l, m, n;          /* interior lattice sizes (lx, ly, lz of the serial deck) */
+    int32_t mode;             /* EUP_MODE_* */
+    int32_t storage;          /* EUP_STORE_* */
+    int32_t front, rear;      /* x = l+1 and x = 0 face codes (storage.F90 front(1), rear(1)) */
+    int32_t up, down, left, right; /* z and y faces: must be EUP_FACE_PERIODIC */
+    int32_t device;           /* HIP device ordinal */
+    uint32_t flags;           /* EUP_FLAG_* */
+    double omega;             /* 2/(6*svisc + 1), computed by the caller as hencol.F90:30 */
+    double fgrad;             /* x body-force coefficient (hencol.F90:37; 0 for CYLINDER_INF) */
+    double u_inflow;          /* inlet velocity (build.bcond.F90:324) */
+} eup_config;
+
+typedef struct eup_info {
+    int64_t sites;            /* l*m*n */
+    int64_t nsolid;           /* solid sites */
+    int64_t tiles, tiles_fluid, tiles_mixed, tiles_solid; /* 64-site tiles (physics mode) */
+    int32_t bbox[6];          /* imin, imax, jmin, jmax, kmin, kmax (build.obs.F90:155-163) */
+    int32_t bytes_per_site;   /* algorithmic HBM bytes per updated site per step */
+    int64_t device_bytes;     /* device memory held by the handle */
+} eup_info;
+
+/* --- lifetime ------------------------------------------------------------------------------ */
+
+/* Replaces setup/alloca/build_bcond/hencol bookkeeping (setup.F90:22, alloca.F90:91-133,
+ * build.bcond.F90:319-339): validates the configuration and allocates the device state. */
+int eup_create(const eup_config *cfg, eup_handle **out);
+
+/* Releases every device allocation of the handle (dealloca.F90). */
+void eup_destroy(eup_handle *h);
+
+/* Message of the last error on this handle (h == NULL: last error of the calling thread). */
+const char *eup_last_error(const eup_handle *h);
+
+/* --- state --------------------------------------------------------------------------------- */
+
+/* Solid mask obs(1:l,1:m,1:n), int32, i fastest, 1 = solid -- the array build_obs fills
+ * (build.obs.F90:147-186).  Computes the obstacle bounding box and the tile classes. */
+int eup_set_mask(eup_handle *h, const int32_t *obs);
+
+/* Population q (1..19) as a ghosted FP64 host array: the record restore_raw reads
+ * (restore_raw.F90:43-63).  In physics mode only the interior is used. */
+int eup_set_pops(eup_handle *h, int32_t q, const double *a);
+
+/* Population q (1..19) into a ghosted FP64 host array: the record save_raw writes
+ * (save_raw.F90:54-74).  Physics mode writes 0 into the ghost layer. */
+int eup_get_pops(eup_handle *h, int32_t q, double *a);
+
+/* All 19 populations at once: a[(q-1)*(l+2)(m+2)(n+2) + offset]. */
+int eup_set_state(eup_handle *h, const double *a);
+int eup_get_state(eup_handle *h, double *a);
+
+/* --- time stepping ------------------------------------------------------------------------- */
+
+/* nsteps iterations of `boundaries; propagation; collision` (bgk3d.F90:128-138). */
+int eup_step(eup_handle *h, int32_t nsteps);
+
+/* As eup_step, and reports the GPU time of the nsteps (HIP events on the handle's stream). */
+int eup_step_timed(eup_handle *h, int32_t nsteps, double *gpu_ms);
+
+/* Blocks until all work on the handle's stream is done. */
+int eup_sync(eup_handle *h);
+
+/* --- diagnostics (FP64) -------------------------------------------------------------------- */
+
+/* Control-volume stress drag of flux_X/flux_Y (flux_X.F90:50-301, flux_Y.F90:51-282) on the box
+ * {xstart, xstop, ystart, ystop, zstart, zstop} (global 1-based indices, drag.F90:48-58).  Per-site
+ * stresses are computed on the GPU and summed in the reference's loop order, so the result equals
+ * the reference's to the last bit on the same populations. */
+int eup_drag_cv(eup_handle *h, const int32_t box[6], double *fluxX, double *fluxY);
+
+/* Momentum-exchange force on the solid, F = sum over fluid sites x and links p with x + c_p solid
+ * of 2 c_p f_p(x), from the current (post-collision) populations; wavefront-reduced in FP64.
+ * No reference counterpart (the reference only has eup_drag_cv's control-volume drag). */
+int eup_drag_mem(eup_handle *h, double F[3]);
+
+/* Velocity probe of probe_global.F90:41-77 at interior site (i, j, k): out = {u, v, w, rho}. */
+int eup_probe(eup_handle *h, int32_t i, int32_t j, int32_t k, double out[4]);
+
+/* Global means of diagno.F90:53-131: out = {<rho>, <u>, <v>, <w>, <u^2>}, solid sites counted as
+ * rho = 1, u = 0.  FP64 reduction on the GPU (fixed order; not the reference's sequential order). */
+int eup_diagno(eup_handle *h, double out[5]);
+
+/* --- introspection ------------------------------------------------------------------------- */
+
+int eup_get_info(eup_handle *h, eup_info *info);
+
+/* The handle's HIP stream (hipStream_t) as an opaque pointer, for callers that order their own
+ * work (timing events, PyTorch) with the library's. */
+void *eup_stream(eup_handle *h);
+
+/* ABI version of the loaded library (EUP_ABI_VERSION at build time). */
+int32_t eup_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EUP_H */

@@ -93,3 +93,37 @@ def _read_mlups(path):
                 if nums:
                     return float(nums[-1].replace("D", "E").replace("d", "e"))
     return None
+
+
+def time_reference(l, m, n, svisc, u0, steps, mask, *, threads=None, variant="omp", timeout=900):
+    """Time ``steps`` steps of the reference's OpenMP build on a lattice started from rest.
+
+    The reference's own initial condition (init_v = 0, CODE/SRC/init.F90:128-236) avoids writing
+    a multi-GB restore file; the step cost does not depend on the values.  I/O intervals are set
+    beyond the last step so that the loop time holds no output.  Returns the reference's MLUPS
+    (bgk.perf, CODE/SRC/finalize.F90:107-108) and the thread count used.
+    """
+    exe = ref_binary(variant)
+    if exe is None:
+        raise FileNotFoundError(f"reference build {variant!r} missing")
+    if threads is None:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    wd = tempfile.mkdtemp(prefix="eupref_t_")
+    try:
+        big = steps + 10
+        fortio.write_deck(os.path.join(wd, "bgk.input"), svisc=float(svisc), u0=float(u0),
+                          itfin=int(steps), ivtim=big, isignal=big, itsave=big, icheck=big,
+                          irestart=0, init_v=0, lx=l, ly=m, lz=n, proc_x=1, proc_y=1, proc_z=1,
+                          tstep=1, flag1=1, flag2=2, flag3=0)
+        fortio.write_obs(os.path.join(wd, fortio.obs_file_name()), mask)
+        env = dict(os.environ)
+        env["OMP_NUM_THREADS"] = str(threads)
+        env.setdefault("OMP_PROC_BIND", "close")
+        proc = subprocess.run([exe], cwd=wd, env=env, capture_output=True, text=True,
+                              timeout=timeout)
+        if proc.returncode != 0:
+            raise RuntimeError(f"reference exited {proc.returncode}: {proc.stdout[-1000:]}")
+        return {"mlups": _read_mlups(os.path.join(wd, "bgk.perf")), "threads": threads,
+                "steps": steps}
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)

@@ -1,0 +1,206 @@
+"""GPU parity: libeup (through its C-ABI) against the golden fixtures and the oracle.
+
+Parity mode must reproduce the reference's serial FP64 step bit for bit (every site, ghosts
+included).  Physics mode in FP64 must equal the oracle's physics step bit for bit; reduced
+storage is checked against it with tolerances stated in each test.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from golden_cases import CASES, make_inputs, omega_of, sha, e14_6
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def lattice(rec_or_shape, **kw):
+    from euplectella_hpc_amd.lib import Lattice
+    return Lattice(*rec_or_shape, **kw)
+
+
+def announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def parity_lattice(name, golden):
+    rec = golden[name]
+    mask, pops = make_inputs(CASES[name])
+    lat = lattice((rec["l"], rec["m"], rec["n"]), svisc=rec["svisc"], u_inflow=rec["u_in"],
+                  mode="parity", storage="f64")
+    lat.set_mask(mask)
+    lat.set_state(pops)
+    return rec, mask, pops, lat
+
+
+def test_parity_tiny_full_state(golden):
+    announce("parity c0_tiny 16x8x12, 20 steps")
+    rec, mask, pops, lat = parity_lattice("c0_tiny", golden)
+    with lat:
+        lat.step(rec["steps"])
+        got = lat.get_state()
+    ref = np.load(os.path.join(GOLDEN, "c0_tiny_state.npz"))[f"t{rec['steps']}"]
+    bad = got.view(np.uint64) != ref.view(np.uint64)
+    assert not bad.any(), f"{bad.sum()} values differ (first at {np.argwhere(bad)[:3].tolist()})"
+
+
+@pytest.mark.parametrize("name", ["c1_small", "c2_ridged", "c3_sponge", "c1_cylinder"])
+def test_parity_state_hashes(golden, name):
+    rec, mask, pops, lat = parity_lattice(name, golden)
+    announce(f"parity {name} {rec['l']}x{rec['m']}x{rec['n']} up to {rec['steps']} steps")
+    done = 0
+    with lat:
+        for t in sorted(int(s) for s in rec["state_sha256"]):
+            lat.step(t - done)
+            done = t
+            got = lat.get_state()
+            assert sha(got) == rec["state_sha256"][str(t)], f"{name}: state differs at step {t}"
+
+
+@pytest.mark.parametrize("name,nsteps", [("c1_small", 10), ("c2_ridged", 60)])
+def test_parity_probe_records(golden, name, nsteps):
+    rec, mask, pops, lat = parity_lattice(name, golden)
+    announce(f"parity probe {name}")
+    i, j, k = rec["probe_site"]
+    with lat:
+        for row in rec["probe"][:nsteps]:
+            lat.step(1)
+            got = lat.probe(i, j, k)
+            assert [e14_6(v) for v in got] == [e14_6(v) for v in row[1:5]], f"step {row[0]}"
+
+
+def test_parity_drag_records(golden):
+    rec, mask, pops, lat = parity_lattice("drag_r100", golden)
+    announce("parity drag_r100 680x340x4, 3 steps, control-volume drag every step")
+    rows = iter(rec["drag"])
+    from euplectella_hpc_amd.lib import drag_boxes
+    with lat:
+        for step in range(1, rec["steps"] + 1):
+            lat.step(1)
+            for delta, box in drag_boxes(rec["l"], rec["m"], rec["n"]):
+                d, t, fx, fy = next(rows)
+                assert (int(d), int(t)) == (delta, step)
+                gx, gy = lat.drag_cv(box)
+                assert gx == fx and gy == fy, f"delta {delta} step {step}: {gx} {gy} vs {fx} {fy}"
+        assert sha(lat.get_state()) == rec["state_sha256"][str(rec["steps"])]
+
+
+# ----------------------------------------------------------------------------------------------
+# physics mode
+# ----------------------------------------------------------------------------------------------
+
+def physics_case(kind, l, m, n, seed, u=0.05):
+    from euplectella_hpc_amd import geometry
+    if kind == "cylinder":
+        mask = geometry.cylinder(l, m, n, max(2, m // 5), (l // 4, m // 2))
+    elif kind == "sponge":
+        mask = geometry.sponge(l, m, n, m // 4, (l // 4, m // 2), cell=8, seed=seed)
+    else:
+        mask = np.zeros((n, m, l), np.uint8)
+    pops = geometry.initial_state(l, m, n, u, seed=seed)
+    return mask, pops
+
+
+def interior(p):
+    return p[:, 1:-1, 1:-1, 1:-1]
+
+
+@pytest.mark.parametrize("kind,x_open,shape", [("cylinder", True, (70, 24, 20)),
+                                                ("cylinder", False, (64, 24, 20)),
+                                                ("sponge", True, (128, 40, 48))])
+def test_physics_f64_equals_oracle(kind, x_open, shape):
+    l, m, n = shape
+    announce(f"physics f64 {kind} {shape} x_open={x_open} vs oracle, 25 steps")
+    mask, pops = physics_case(kind, l, m, n, seed=7)
+    svisc = (2 * 0.55 - 1) / 6
+    ref = pops.copy()
+    orc.physics_steps(ref, mask, orc.omega_from_svisc(svisc), 0.05, 25, x_open=x_open)
+    with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="physics", storage="f64",
+                 x_open=x_open) as lat:
+        lat.set_mask(mask)
+        lat.set_state(pops)
+        lat.step(25)
+        got = lat.get_state()
+    a, b = interior(got), interior(ref)
+    bad = a.view(np.uint64) != b.view(np.uint64)
+    assert not bad.any(), f"{bad.sum()} values differ, max abs {np.abs(a - b).max()}"
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
+def test_physics_tile_skip_is_exact(storage):
+    l, m, n = 128, 48, 48
+    announce(f"physics {storage} sponge {l}x{m}x{n}: tile skip vs no skip")
+    mask, pops = physics_case("sponge", l, m, n, seed=3)
+    svisc = (2 * 0.52 - 1) / 6
+    out = []
+    for skip in (True, False):
+        with lattice((l, m, n), svisc=svisc, u_inflow=0.06, mode="physics", storage=storage,
+                     tile_skip=skip) as lat:
+            lat.set_mask(mask)
+            lat.set_state(pops)
+            lat.step(20)
+            out.append(lat.get_state())
+            if skip:
+                info = lat.info()
+                assert info["tiles_mixed"] > 0 and info["tiles_fluid"] > 0
+    assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 2e-6), ("f16", 2e-4), ("bf16", 2e-3)])
+def test_physics_reduced_storage_close_to_f64(storage, tol):
+    """Populations after 40 steps within ``tol`` (absolute; populations are O(1/36)) of FP64."""
+    l, m, n = 96, 32, 32
+    announce(f"physics {storage} vs oracle f64, cylinder {l}x{m}x{n}, 40 steps")
+    mask, pops = physics_case("cylinder", l, m, n, seed=5)
+    svisc = (2 * 0.6 - 1) / 6
+    ref = pops.copy()
+    orc.physics_steps(ref, mask, orc.omega_from_svisc(svisc), 0.05, 40)
+    with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="physics", storage=storage) as lat:
+        lat.set_mask(mask)
+        lat.set_state(pops)
+        lat.step(40)
+        got = lat.get_state()
+    fluid = mask == 0
+    err = np.abs(interior(got) - interior(ref))[:, fluid].max()
+    assert err < tol, f"{storage}: max abs population error {err}"
+
+
+def test_physics_drag_mem_and_diagno_match_oracle():
+    l, m, n = 96, 32, 24
+    announce(f"physics f64 drag_mem / diagno / probe vs oracle {l}x{m}x{n}")
+    mask, pops = physics_case("cylinder", l, m, n, seed=9)
+    svisc = (2 * 0.6 - 1) / 6
+    ref = pops.copy()
+    orc.physics_steps(ref, mask, orc.omega_from_svisc(svisc), 0.05, 30)
+    with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="physics", storage="f64") as lat:
+        lat.set_mask(mask)
+        lat.set_state(pops)
+        lat.step(30)
+        F = lat.drag_mem()
+        d = lat.diagno()
+        p = lat.probe(40, 20, 5)
+    Fr = orc.mem_force(ref, mask)
+    # summation order differs (wavefront tree vs sequential): absolute bound scaled by |F|
+    assert np.abs(F - Fr).max() <= 1e-12 * np.abs(Fr).max(), (F, Fr)
+    dr = orc.diagno(ref, mask)
+    assert np.allclose(d, dr, rtol=1e-13, atol=1e-16), (d, dr)
+    pr = orc.probe(ref, 40, 20, 5)
+    assert np.array_equal(p, pr)
+
+
+def test_parity_diagno_and_mem_match_oracle(golden):
+    rec, mask, pops, lat = parity_lattice("c1_small", golden)
+    announce("parity c1_small diagno / drag_mem vs oracle")
+    with lat:
+        lat.step(10)
+        d = lat.diagno()
+        F = lat.drag_mem()
+        got = lat.get_state()
+    assert np.allclose(d, orc.diagno(got, mask), rtol=1e-13, atol=1e-16)
+    Fr = orc.mem_force(got, mask)
+    assert np.abs(F - Fr).max() <= 1e-12 * np.abs(Fr).max(), (F, Fr)
