@@ -28,6 +28,29 @@ def hipcc():
     return "hipcc"
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_EXE = os.path.join(FORTRAN_DIR, "bgk3d_eup.x")
+
+
+def build_fortran(verbose=False):
+    """Fortran host (ISO_C_BINDING) linked against libeup.so; needs amdflang."""
+    fc = "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        if verbose:
+            print("amdflang not found: Fortran host not built")
+        return None
+    cmd = [fc, "-O2", "-module-dir", FORTRAN_DIR,
+           os.path.join(FORTRAN_DIR, "eup_mod.F90"), os.path.join(FORTRAN_DIR, "bgk3d_eup.F90"),
+           "-L" + HERE, "-leup", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_EXE]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("amdflang failed building the Fortran host")
+    if verbose:
+        print("built", FORTRAN_EXE)
+    return FORTRAN_EXE
+
+
 def build(verbose=False):
     srcs = [os.path.join(HERE, "csrc", s) for s in SOURCES]
     cmd = [hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include")] + srcs + ["-o", OUT + ".tmp"]
@@ -38,6 +61,7 @@ def build(verbose=False):
     os.replace(OUT + ".tmp", OUT)
     if verbose:
         print("built", OUT)
+    build_fortran(verbose)
     return OUT
 
 

@@ -1,0 +1,206 @@
+! Fortran host for the GPU time step: the reference's serial run flow with libeup underneath.
+!
+! Keeps the reference's files and formats unchanged:
+!   bgk.input               namelist &parameters (CODE/SRC/input.F90:46-62)
+!   obs.0000.00000000.bin   obstacle list, flag1 = 1 (build.obs.F90:147-186)
+!   restore.00.00.00.bin    raw restart, irestart = 1 (restore_raw.F90:43-63)
+!   save.00.00.00.T.bin     raw dump every itsave steps (save_raw.F90:54-74)
+!   drag.0000.dat           control-volume drag every icheck steps (drag.F90:48-67), written when
+!                           EUP_DRAG=1 (the reference's -DDRAG build)
+!   probe_g.0000.dat        velocity probe every icheck steps (probe_global.F90:77,93;
+!                           site from diagnostic.F90:145)
+!   diagno.dat              global means at itime 0 for a fresh start (diagno.F90:157)
+!   bgk.perf                MLUPS line (finalize.F90:107-108,222)
+! Case: CYLINDER_INF + OBSTACLES (inflow at x = 0, outflow at x = l+1, periodic y and z;
+! build.bcond.F90:319-339).  The step semantics and storage come from the environment:
+!   EUP_MODE    = parity (default; the reference's serial FP64 step bit for bit) | physics
+!   EUP_STORAGE = f64 (default) | f32 | f16 | bf16   (physics mode)
+!   EUP_DEVICE  = HIP device ordinal (default 0)
+program bgk3d_eup
+   use iso_c_binding
+   use eup
+   implicit none
+
+   real(c_double) :: svisc, u0, u_inflow, omega, fgrad
+   integer :: itfin, ivtim, isignal, itsave, icheck, irestart, init_v
+   integer :: lx, ly, lz, proc_x, proc_y, proc_z, tstep, flag1, flag2, flag3, ipad, jpad, kpad
+   namelist /parameters/ svisc, u0, itfin, ivtim, isignal, itsave, icheck, irestart, init_v, &
+      lx, ly, lz, proc_x, proc_y, proc_z, tstep, flag1, flag2, flag3, ipad, jpad, kpad
+
+   type(eup_config) :: cfg
+   type(c_ptr) :: h
+   integer(c_int32_t), allocatable :: obs(:, :, :)
+   real(c_double), allocatable :: buf(:)
+   real(c_double) :: pr(4), dg(5), fx, fy, tloop
+   integer(c_int32_t) :: box(6)
+   integer :: l, m, n, q, itime, itstart, nobs, idx, i, j, k, nxt, delta
+   integer(8) :: c0, c1, crate
+   integer :: do_drag
+   integer(c_int) :: rc
+   character(len=32) :: env
+   character(len=27) :: fname
+
+   flag1 = 2; flag2 = 2; flag3 = 1; ipad = 0; jpad = 0; kpad = 0
+   open(15, file='bgk.input', status='old')
+   read(15, parameters)
+   close(15)
+   if (proc_x * proc_y * proc_z /= 1) then
+      write(6,*) "ERROR: this host runs one task (proc_x = proc_y = proc_z = 1)"
+      stop 1
+   end if
+   l = lx; m = ly; n = lz
+   ! build_bcond for CYLINDER_INF: the deck's u0 becomes the inflow velocity, no volume force
+   u_inflow = u0
+   u0 = 0
+   ! hencol
+   omega = 2.d0 / (6.d0 * svisc + 1.d0)
+   fgrad = 4.0 * u0 * svisc / (5.0 * real(n, c_double) * real(n, c_double))
+
+   cfg%l = l; cfg%m = m; cfg%n = n
+   cfg%mode = EUP_MODE_PARITY
+   cfg%storage = EUP_STORE_F64
+   call get_environment_variable('EUP_MODE', env)
+   if (trim(env) == 'physics') cfg%mode = EUP_MODE_PHYSICS
+   call get_environment_variable('EUP_STORAGE', env)
+   select case (trim(env))
+   case ('f32'); cfg%storage = EUP_STORE_F32
+   case ('f16'); cfg%storage = EUP_STORE_F16
+   case ('bf16'); cfg%storage = EUP_STORE_BF16
+   end select
+   cfg%device = 0
+   call get_environment_variable('EUP_DEVICE', env)
+   if (len_trim(env) > 0) read(env, *) cfg%device
+   do_drag = 0
+   call get_environment_variable('EUP_DRAG', env)
+   if (trim(env) == '1') do_drag = 1
+   cfg%front = EUP_FACE_OUTFLOW
+   cfg%rear = EUP_FACE_INFLOW
+   cfg%up = EUP_FACE_PERIODIC; cfg%down = EUP_FACE_PERIODIC
+   cfg%left = EUP_FACE_PERIODIC; cfg%right = EUP_FACE_PERIODIC
+   cfg%flags = 0
+   cfg%omega = omega
+   cfg%fgrad = fgrad
+   cfg%u_inflow = u_inflow
+   rc = eup_create(cfg, h)
+   call eup_check(rc, c_null_ptr, 'eup_create')
+
+   ! obstacles (build_obs, flag1 = 1 reads the task's list; 0 = no obstacle)
+   allocate(obs(l, m, n))
+   obs = 0
+   if (flag1 == 1) then
+      open(53, file='obs.0000.00000000.bin', form='unformatted', status='old')
+      read(53) nobs
+      do idx = 1, nobs
+         read(53) i, j, k
+         obs(i, j, k) = 1
+      end do
+      close(53)
+   else if (flag1 /= 0) then
+      write(6,*) "ERROR: flag1 must be 0 or 1 (obstacle list file)", flag1
+      stop 1
+   end if
+   rc = eup_set_mask(h, obs)
+   call eup_check(rc, h, 'eup_set_mask')
+
+   ! populations: restore_raw or the rest state of init opt 0
+   allocate(buf((l + 2) * (m + 2) * (n + 2)))
+   if (irestart == 1) then
+      open(30, file='restore.00.00.00.bin', form='unformatted', status='old')
+      read(30) itstart
+      do q = 1, 19
+         read(30) buf
+         rc = eup_set_pops(h, q, buf)
+         call eup_check(rc, h, 'eup_set_pops')
+      end do
+      close(30)
+   else
+      if (init_v /= 0) then
+         write(6,*) "ERROR: this host starts from rest (init_v = 0) or from a restore file"
+         stop 1
+      end if
+      itstart = 0
+      do q = 1, 19
+         buf = weight(q)
+         rc = eup_set_pops(h, q, buf)
+         call eup_check(rc, h, 'eup_set_pops')
+      end do
+      rc = eup_diagno(h, dg)
+      call eup_check(rc, h, 'eup_diagno')
+      open(63, file='diagno.dat', status='unknown')
+      write(63, 1004) 0, dg(2), dg(3), dg(4), dg(1), dg(5)
+      close(63)
+   end if
+
+   open(66, file='drag.0000.dat', status='unknown')
+   open(67, file='probe_g.0000.dat', status='unknown')
+
+   call system_clock(c0, crate)
+   itime = itstart
+   do while (itime < itfin)
+      nxt = min(itfin, (itime / icheck + 1) * icheck, (itime / itsave + 1) * itsave)
+      rc = eup_step(h, nxt - itime)
+      call eup_check(rc, h, 'eup_step')
+      itime = nxt
+      if (mod(itime, icheck) == 0) then
+         if (do_drag == 1) then
+            do delta = 52, 67, 5
+               box = [lx / 4 - 100 - delta, lx / 4 + 100 + delta, ly / 2 - 100 - delta, &
+                      ly / 2 + 100 + delta, 1, lz]
+               rc = eup_drag_cv(h, box, fx, fy)
+               call eup_check(rc, h, 'eup_drag_cv')
+               write(66, *) delta, itime, fx, fy, 0
+            end do
+         end if
+         i = 8 * lx / 20; j = 9 * ly / 16; k = lz / 2 - 5
+         if (i >= 1 .and. i <= l .and. j >= 1 .and. j <= m .and. k >= 1 .and. k <= n) then
+            rc = eup_probe(h, i, j, k, pr)
+            call eup_check(rc, h, 'eup_probe')
+            write(67, 1002) itime, pr(1), pr(2), pr(3), pr(4)
+         end if
+      end if
+      if (mod(itime, itsave) == 0) then
+         fname = 'save.00.00.00.xxxxxxxx.bin'
+         write(fname(15:22), '(i8.8)') itime
+         open(20, file=fname, form='unformatted', status='unknown')
+         write(20) itime
+         do q = 1, 19
+            rc = eup_get_pops(h, q, buf)
+            call eup_check(rc, h, 'eup_get_pops')
+            write(20) buf
+         end do
+         close(20)
+      end if
+   end do
+   call system_clock(c1)
+   tloop = real(c1 - c0, c_double) / real(crate, c_double)
+   close(66)
+   close(67)
+
+   open(69, file='bgk.perf', status='unknown')
+   write(69, *) "# Run info (libeup)"
+   write(69, *) lx, ly, lz
+   write(69, *) "# Derived (global) metrics "
+   if (tloop > 0) then
+      write(69, 1106) float(lx) * float(ly) * float(lz) * (itfin - itstart) / tloop / 1000.0 / 1000.0
+   end if
+   close(69)
+   call eup_destroy(h)
+   write(6,*) "That's all folks!!!!"
+
+1002 format(i8, 4(e14.6, 1x))
+1004 format(i8, 5(e14.6, 1x))
+1106 format(" # Mlups      ", 1(f14.6, 1x))
+
+contains
+
+   ! D3Q19 weight of population q (storage.F90:77-81; rest state of init.F90 opt 0)
+   real(c_double) function weight(q)
+      integer, intent(in) :: q
+      select case (q)
+      case (19); weight = 1.d0 / 3.d0
+      case (5, 6, 8, 14, 15, 17); weight = 1.d0 / 18.d0
+      case default; weight = 1.d0 / 36.d0
+      end select
+   end function weight
+
+end program bgk3d_eup

@@ -1,0 +1,135 @@
+! ISO_C_BINDING interface to libeup.so (include/eup.h) for the sponge-flow Fortran host.
+!
+! The reference calls its time step as argument-less subroutines over module state
+! (CODE/SRC/bgk3d.F90:128-138 with `use storage`); with this module the same host keeps its deck,
+! obstacle and restore formats and calls the GPU library through a handle instead.  Errors come
+! back as integer codes; eup_check maps them to the reference's "write(6,*) ERROR; stop" style.
+module eup
+   use iso_c_binding
+   implicit none
+
+   integer(c_int32_t), parameter :: EUP_STORE_F64 = 0, EUP_STORE_F32 = 1
+   integer(c_int32_t), parameter :: EUP_STORE_F16 = 2, EUP_STORE_BF16 = 3
+   integer(c_int32_t), parameter :: EUP_MODE_PARITY = 0, EUP_MODE_PHYSICS = 1
+   integer(c_int32_t), parameter :: EUP_FACE_PERIODIC = 0, EUP_FACE_INFLOW = 4
+   integer(c_int32_t), parameter :: EUP_FACE_OUTFLOW = 5
+   integer(c_int32_t), parameter :: EUP_FLAG_NO_TILE_SKIP = 1
+
+   ! mirrors struct eup_config (flags is uint32_t in C; same size and alignment)
+   type, bind(c) :: eup_config
+      integer(c_int32_t) :: l, m, n
+      integer(c_int32_t) :: mode, storage
+      integer(c_int32_t) :: front, rear, up, down, left, right
+      integer(c_int32_t) :: device
+      integer(c_int32_t) :: flags
+      real(c_double) :: omega, fgrad, u_inflow
+   end type eup_config
+
+   interface
+      integer(c_int) function eup_create(cfg, h) bind(c, name='eup_create')
+         import :: c_int, c_ptr, eup_config
+         type(eup_config), intent(in) :: cfg
+         type(c_ptr), intent(out) :: h
+      end function eup_create
+
+      subroutine eup_destroy(h) bind(c, name='eup_destroy')
+         import :: c_ptr
+         type(c_ptr), value :: h
+      end subroutine eup_destroy
+
+      type(c_ptr) function eup_last_error(h) bind(c, name='eup_last_error')
+         import :: c_ptr
+         type(c_ptr), value :: h
+      end function eup_last_error
+
+      integer(c_int) function eup_set_mask(h, obs) bind(c, name='eup_set_mask')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), intent(in) :: obs(*)
+      end function eup_set_mask
+
+      integer(c_int) function eup_set_pops(h, q, a) bind(c, name='eup_set_pops')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: q
+         real(c_double), intent(in) :: a(*)
+      end function eup_set_pops
+
+      integer(c_int) function eup_get_pops(h, q, a) bind(c, name='eup_get_pops')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: q
+         real(c_double), intent(out) :: a(*)
+      end function eup_get_pops
+
+      integer(c_int) function eup_step(h, nsteps) bind(c, name='eup_step')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: nsteps
+      end function eup_step
+
+      integer(c_int) function eup_step_timed(h, nsteps, ms) bind(c, name='eup_step_timed')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: nsteps
+         real(c_double), intent(out) :: ms
+      end function eup_step_timed
+
+      integer(c_int) function eup_drag_cv(h, box, fx, fy) bind(c, name='eup_drag_cv')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), intent(in) :: box(6)
+         real(c_double), intent(out) :: fx, fy
+      end function eup_drag_cv
+
+      integer(c_int) function eup_drag_mem(h, f) bind(c, name='eup_drag_mem')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(out) :: f(3)
+      end function eup_drag_mem
+
+      integer(c_int) function eup_probe(h, i, j, k, out) bind(c, name='eup_probe')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: i, j, k
+         real(c_double), intent(out) :: out(4)
+      end function eup_probe
+
+      integer(c_int) function eup_diagno(h, out) bind(c, name='eup_diagno')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(out) :: out(5)
+      end function eup_diagno
+
+      integer(c_int32_t) function eup_abi_version() bind(c, name='eup_abi_version')
+         import :: c_int32_t
+      end function eup_abi_version
+   end interface
+
+contains
+
+   ! Stop with the library's message when rc /= 0 (the reference reports errors with
+   ! write(6,*) and stop, e.g. CODE/SRC/input.F90:88-101).
+   subroutine eup_check(rc, h, what)
+      integer(c_int), intent(in) :: rc
+      type(c_ptr), intent(in) :: h
+      character(len=*), intent(in) :: what
+      character(kind=c_char), pointer :: msg(:)
+      type(c_ptr) :: p
+      integer :: i
+      character(len=512) :: text
+      if (rc == 0) return
+      p = eup_last_error(h)
+      text = ' '
+      if (c_associated(p)) then
+         call c_f_pointer(p, msg, [512])
+         do i = 1, 512
+            if (msg(i) == c_null_char) exit
+            text(i:i) = msg(i)
+         end do
+      end if
+      write(6,*) "ERROR: ", what, " failed (", rc, "): ", trim(text)
+      stop 1
+   end subroutine eup_check
+
+end module eup

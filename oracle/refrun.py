@@ -64,6 +64,8 @@ def run_reference(l, m, n, svisc, u0, steps, mask, pops0, *, variant="nodrag", i
         out["saves"] = saves
         out["drag"] = _read_table(os.path.join(wd, "drag.0000.dat"), 5)
         out["probe"] = _read_table(os.path.join(wd, "probe_g.0000.dat"), 5)
+        out["drag_text"] = _read_text(os.path.join(wd, "drag.0000.dat"))
+        out["probe_text"] = _read_text(os.path.join(wd, "probe_g.0000.dat"))
         out["mlups"] = _read_mlups(os.path.join(wd, "bgk.perf"))
         return out
     finally:
@@ -81,6 +83,13 @@ def _read_table(path, ncol):
             if len(parts) >= ncol - 1:
                 rows.append(tuple(float(p) for p in parts[:ncol]))
     return rows
+
+
+def _read_text(path):
+    if not os.path.exists(path):
+        return ""
+    with open(path) as fh:
+        return fh.read()
 
 
 def _read_mlups(path):

@@ -88,8 +88,10 @@ def main(names=None):
         if case.get("probe"):
             rec["probe_site"] = [8 * l // 20, 9 * m // 16, n // 2 - 5]
             rec["probe"] = [list(r) for r in res["probe"]]
+            rec["probe_text"] = res["probe_text"]
         if drag:
             rec["drag"] = [list(r[:4]) for r in res["drag"]]
+            rec["drag_text"] = res["drag_text"]
         if case.get("full_state"):
             last = saves[-1]
             np.savez_compressed(os.path.join(OUT, f"{name}_state.npz"),
