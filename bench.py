@@ -113,6 +113,29 @@ def cpu_baseline(wl, steps=30, nz=64):
                       f"sponge, {steps} steps, MLUPS from its bgk.perf"}
 
 
+def copy_probe(mib, reps=10):
+    """Plain device-to-device copy of ``mib`` MiB (torch), the HBM reference for the roofline and
+    the calibration kernel for the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    gbs = 2 * a.numel() * 4 / (ms / 1e3) / 1e9
+    print(json.dumps({"copy_mib": mib, "bytes_read": a.numel() * 4, "bytes_written": a.numel() * 4,
+                      "ms_per_copy": round(ms, 4), "gbs": round(gbs, 1)}))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,7 +145,11 @@ def main(argv=None):
     ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16", "bf16"])
     ap.add_argument("--secondary", default="f16", help="second storage to report ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain MIB-MiB device copy (bandwidth / counter calibration)")
     args = ap.parse_args(argv)
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not supported "
               "(the x-slab halo exchange is a design, not an implementation)", file=sys.stderr)

@@ -22,6 +22,8 @@
  *   orc_physics_steps  the same collision and boundary arithmetic with y/z periodicity applied by
  *                      index wrap (ghosts fresh before bounce-back), solid sites not collided.
  *   orc_mem_force      momentum-exchange force on the solid (no reference counterpart).
+ *   orc_grey_*         grey-lattice (partial bounce-back) step, force, objective and its discrete
+ *                      adjoint with checkpoint-recompute (no reference counterpart).
  *
  * Parity pins (see tests/test_oracle_*.py): bit-equality with the reference executable built by
  * oracle/Makefile.ref on seeded inputs; closed forms (equilibrium moments, rest / uniform-flow
@@ -611,4 +613,323 @@ void orc_diagno(const double *a, const int32_t *obs, int l, int m, int n, double
     out[2] = ytot * rdv;
     out[3] = ztot * rdv;
     out[4] = stot * rdv;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* grey lattice (partial bounce-back) and its discrete adjoint -- no reference counterpart     */
+/*                                                                                            */
+/* Link-wise partial bounce-back: the link between x and its source y = x - c_q carries the   */
+/* fraction sigma = 1 - (1 - s(x))(1 - s(y)) (s = 0 on ghost sites); the pulled population is */
+/*     f_q(x) = (1 - sigma) src_q + sigma a_qbar(x),                                          */
+/* src_q = a_q(y) inside the lattice, the inlet or outlet equilibrium across the x faces.     */
+/* sigma is symmetric in the link, so mass is conserved on every link, and a binary s gives   */
+/* exactly the halfway bounce-back of the physics step at every fluid site.                   */
+/* Force on the grey body: F = sum over interior links (x, q) of sigma c_q (a_q(y) - a_qbar(x)) */
+/* (each undirected link is met twice, once from each end).                                   */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+    int l, m, n;
+    double omega, u_in;
+    long N;
+    int xopen; /* 1: inlet at x = 0, outlet at x = l+1; 0: periodic x */
+} grey_t;
+
+#define SIDX(G, i, j, k) (((long)(i) - 1) + (long)(G).l * (((long)(j) - 1) + (long)(G).m * ((long)(k) - 1)))
+
+/* pulled populations of site (i,j,k) (1-based, interior) from the dense interior state a[19][N];
+ * sig[q] and kind[q] (0 interior source, 1 inlet, 2 outlet) describe each link */
+static void grey_pull(const double *a, const double *s, grey_t G, int i, int j, int k, double *f,
+                      double *sig, int *kind, double *srcv, long *srcidx)
+{
+    double inl[5];
+    inlet_eq(G.u_in, inl);
+    const long xs = SIDX(G, i, j, k);
+    const double sx = s[xs];
+    for (int q = 0; q < 18; ++q) {
+        int si = i - CX[q], sj = wrap(j - CY[q], G.m), sk = wrap(k - CZ[q], G.n);
+        double src, sy;
+        if (!G.xopen) si = wrap(si, G.l);
+        if (si < 1) {
+            src = inl[q];
+            sy = 0.0;
+            kind[q] = 1;
+            srcidx[q] = -1;
+        } else if (si > G.l) {
+            double y[19], o[5];
+            long ys = SIDX(G, G.l, sj, sk);
+            for (int p = 0; p < 19; ++p) y[p] = a[p * G.N + ys];
+            outlet_eq(y, o);
+            src = o[q - 9];
+            sy = 0.0;
+            kind[q] = 2;
+            srcidx[q] = ys;
+        } else {
+            long ys = SIDX(G, si, sj, sk);
+            src = a[q * G.N + ys];
+            sy = s[ys];
+            kind[q] = 0;
+            srcidx[q] = ys;
+        }
+        double sg = 1.0 - (1.0 - sx) * (1.0 - sy);
+        sig[q] = sg;
+        srcv[q] = src;
+        f[q] = (1.0 - sg) * src + sg * a[OPP[q] * G.N + xs];
+    }
+    f[18] = a[18 * G.N + xs];
+}
+
+static void grey_step(const double *a, double *b, const double *s, grey_t G)
+{
+    double f[19], sig[18], srcv[18];
+    int kind[18];
+    long sidx[18];
+    for (int k = 1; k <= G.n; ++k)
+        for (int j = 1; j <= G.m; ++j)
+            for (int i = 1; i <= G.l; ++i) {
+                grey_pull(a, s, G, i, j, k, f, sig, kind, srcv, sidx);
+                collide_site(f, f, G.omega, 0.0);
+                long xs = SIDX(G, i, j, k);
+                for (int q = 0; q < 19; ++q) b[q * G.N + xs] = f[q];
+            }
+}
+
+static void grey_force(const double *a, const double *s, grey_t G, double F[3])
+{
+    double f[19], sig[18], srcv[18];
+    int kind[18];
+    long sidx[18];
+    double fx = 0, fy = 0, fz = 0;
+    for (int k = 1; k <= G.n; ++k)
+        for (int j = 1; j <= G.m; ++j)
+            for (int i = 1; i <= G.l; ++i) {
+                grey_pull(a, s, G, i, j, k, f, sig, kind, srcv, sidx);
+                long xs = SIDX(G, i, j, k);
+                for (int q = 0; q < 18; ++q) {
+                    if (kind[q] != 0) continue;
+                    double d = sig[q] * (srcv[q] - a[OPP[q] * G.N + xs]);
+                    fx += CX[q] * d;
+                    fy += CY[q] * d;
+                    fz += CZ[q] * d;
+                }
+            }
+    F[0] = fx; F[1] = fy; F[2] = fz;
+}
+
+/* (dC/df)^T lam for the BGK collision at pre-collision f (no body force) */
+static void collide_adj(const double *f, const double *lam, double omega, double *out)
+{
+    double rho = 0, jx = 0, jy = 0, jz = 0;
+    for (int q = 0; q < 19; ++q) {
+        rho += f[q];
+        jx += CX[q] * f[q];
+        jy += CY[q] * f[q];
+        jz += CZ[q] * f[q];
+    }
+    double ux = jx / rho, uy = jy / rho, uz = jz / rho, usq = ux * ux + uy * uy + uz * uz;
+    double A = 0, Bx = 0, By = 0, Bz = 0;
+    for (int q = 0; q < 19; ++q) {
+        double w = q == 18 ? P0 : ((CX[q] * CX[q] + CY[q] * CY[q] + CZ[q] * CZ[q]) == 1 ? P1 : P2);
+        double cu = CX[q] * ux + CY[q] * uy + CZ[q] * uz;
+        A += lam[q] * w * (1.0 - 4.5 * cu * cu + 1.5 * usq);
+        double g = lam[q] * w * (3.0 + 9.0 * cu);
+        Bx += g * CX[q] - lam[q] * w * 3.0 * ux;
+        By += g * CY[q] - lam[q] * w * 3.0 * uy;
+        Bz += g * CZ[q] - lam[q] * w * 3.0 * uz;
+    }
+    for (int p = 0; p < 19; ++p)
+        out[p] = (1.0 - omega) * lam[p] + omega * (A + Bx * CX[p] + By * CY[p] + Bz * CZ[p]);
+}
+
+/* d OUT_q / d a_p of the outlet equilibrium (momentum J of the source site, rho fixed to 1) */
+static double outlet_grad(const double *y, int q, int p)
+{
+    double Jx = 0, Jy = 0, Jz = 0;
+    for (int r = 0; r < 19; ++r) {
+        Jx += CX[r] * y[r];
+        Jy += CY[r] * y[r];
+        Jz += CZ[r] * y[r];
+    }
+    double w = ((CX[q] * CX[q] + CY[q] * CY[q] + CZ[q] * CZ[q]) == 1) ? P1 : P2;
+    double cJ = CX[q] * Jx + CY[q] * Jy + CZ[q] * Jz;
+    double cc = CX[q] * CX[p] + CY[q] * CY[p] + CZ[q] * CZ[p];
+    double Jc = Jx * CX[p] + Jy * CY[p] + Jz * CZ[p];
+    return w * (3.0 * cc + 9.0 * cJ * cc - 3.0 * Jc);
+}
+
+/* one backward step: lam_out = (d a^{t+1} / d a^t)^T lam_in (+ wobj dF/da), gs += ..., at a = a^t */
+static void grey_step_adj(const double *a, const double *s, grey_t G, const double *lam_in,
+                          double *lam_out, double *lin, double wobj, double *gs)
+{
+    double f[19], sig[18], srcv[18], lam[19], tmp[19];
+    int kind[18];
+    long sidx[18];
+    memset(lam_out, 0, sizeof(double) * 19 * G.N);
+    for (int k = 1; k <= G.n; ++k)
+        for (int j = 1; j <= G.m; ++j)
+            for (int i = 1; i <= G.l; ++i) {
+                long xs = SIDX(G, i, j, k);
+                grey_pull(a, s, G, i, j, k, f, sig, kind, srcv, sidx);
+                for (int q = 0; q < 19; ++q) lam[q] = lam_in[q * G.N + xs];
+                collide_adj(f, lam, G.omega, tmp);
+                for (int q = 0; q < 19; ++q) lin[q * G.N + xs] = tmp[q];
+            }
+    for (int k = 1; k <= G.n; ++k)
+        for (int j = 1; j <= G.m; ++j)
+            for (int i = 1; i <= G.l; ++i) {
+                long xs = SIDX(G, i, j, k);
+                grey_pull(a, s, G, i, j, k, f, sig, kind, srcv, sidx);
+                const double sx = s[xs];
+                for (int q = 0; q < 18; ++q) {
+                    const double L = lin[q * G.N + xs];
+                    const double sg = sig[q];
+                    const double abar = a[OPP[q] * G.N + xs];
+                    lam_out[OPP[q] * G.N + xs] += sg * L;
+                    if (kind[q] == 0) {
+                        long ys = sidx[q];
+                        lam_out[q * G.N + ys] += (1.0 - sg) * L;
+                        double d = abar - srcv[q];
+                        gs[xs] += L * d * (1.0 - s[ys]);
+                        gs[ys] += L * d * (1.0 - sx);
+                        if (wobj != 0.0) {
+                            double c = wobj * CX[q];
+                            lam_out[q * G.N + ys] += c * sg;
+                            lam_out[OPP[q] * G.N + xs] -= c * sg;
+                            gs[xs] += c * (srcv[q] - abar) * (1.0 - s[ys]);
+                            gs[ys] += c * (srcv[q] - abar) * (1.0 - sx);
+                        }
+                    } else {
+                        gs[xs] += L * (abar - srcv[q]);
+                        if (kind[q] == 2) {
+                            double y[19];
+                            long ys = sidx[q];
+                            for (int p = 0; p < 19; ++p) y[p] = a[p * G.N + ys];
+                            for (int p = 0; p < 19; ++p)
+                                lam_out[p * G.N + ys] += (1.0 - sg) * L * outlet_grad(y, q, p);
+                        }
+                    }
+                }
+                lam_out[18 * G.N + xs] += lin[18 * G.N + xs];
+            }
+}
+
+/* objective-only adjoint seed at a^t: lam = wobj dF/da, gs += wobj dF/ds */
+static void grey_force_adj(const double *a, const double *s, grey_t G, double wobj, double *lam,
+                           double *gs)
+{
+    double f[19], sig[18], srcv[18];
+    int kind[18];
+    long sidx[18];
+    for (int k = 1; k <= G.n; ++k)
+        for (int j = 1; j <= G.m; ++j)
+            for (int i = 1; i <= G.l; ++i) {
+                long xs = SIDX(G, i, j, k);
+                grey_pull(a, s, G, i, j, k, f, sig, kind, srcv, sidx);
+                for (int q = 0; q < 18; ++q) {
+                    if (kind[q] != 0) continue;
+                    long ys = sidx[q];
+                    double c = wobj * CX[q];
+                    double abar = a[OPP[q] * G.N + xs];
+                    lam[q * G.N + ys] += c * sig[q];
+                    lam[OPP[q] * G.N + xs] -= c * sig[q];
+                    gs[xs] += c * (srcv[q] - abar) * (1.0 - s[ys]);
+                    gs[ys] += c * (srcv[q] - abar) * (1.0 - s[xs]);
+                }
+            }
+}
+
+/* Dense-interior API: a[19*N] (N = l*m*n, i fastest), s[N]. */
+int orc_grey_steps(double *a, const double *s, int l, int m, int n, double omega, double u_in,
+                   int xopen, int nsteps)
+{
+    grey_t G = {l, m, n, omega, u_in, (long)l * m * n, xopen};
+    double *b = (double *)malloc(sizeof(double) * 19 * G.N);
+    if (!b) return -1;
+    for (int t = 0; t < nsteps; ++t) {
+        grey_step(a, b, s, G);
+        memcpy(a, b, sizeof(double) * 19 * G.N);
+    }
+    free(b);
+    return 0;
+}
+
+void orc_grey_force(const double *a, const double *s, int l, int m, int n, double u_in, int xopen,
+                    double F[3])
+{
+    grey_t G = {l, m, n, 0.0, u_in, (long)l * m * n, xopen};
+    grey_force(a, s, G, F);
+}
+
+/* J = (1/(t1-t0+1)) sum_{t=t0..t1} F_x(a^t) / norm starting from a0 (a^0); gs (may be NULL)
+ * receives dJ/ds.  Forward states are checkpointed every K steps and recomputed per segment. */
+int orc_grey_objective(const double *a0, const double *s, int l, int m, int n, double omega,
+                       double u_in, int xopen, int t0, int t1, int K, double norm, double *J,
+                       double *gs)
+{
+    grey_t G = {l, m, n, omega, u_in, (long)l * m * n, xopen};
+    const size_t S = sizeof(double) * 19 * G.N;
+    if (t0 < 1 || t1 < t0 || K < 1) return -1;
+    const double wobj = 1.0 / ((double)(t1 - t0 + 1) * norm);
+    int nck = t1 / K + 1;
+    double *ck = (double *)malloc(S * nck);
+    double *cur = (double *)malloc(S), *nxt = (double *)malloc(S);
+    if (!ck || !cur || !nxt) return -1;
+    memcpy(cur, a0, S);
+    double acc = 0.0, F[3];
+    for (int t = 0; t <= t1; ++t) {
+        if (t % K == 0) memcpy(ck + (size_t)(t / K) * 19 * G.N, cur, S);
+        if (t >= t0) {
+            grey_force(cur, s, G, F);
+            acc += F[0];
+        }
+        if (t == t1) break;
+        grey_step(cur, nxt, s, G);
+        double *sw = cur; cur = nxt; nxt = sw;
+    }
+    *J = acc * wobj;
+    if (gs) {
+        memset(gs, 0, sizeof(double) * G.N);
+        double *seg = (double *)malloc(S * (size_t)K);
+        double *lam = (double *)calloc(19 * G.N, sizeof(double));
+        double *lam2 = (double *)malloc(S), *lin = (double *)malloc(S);
+        if (!seg || !lam || !lam2 || !lin) return -1;
+        /* seed at t1 */
+        {
+            int c = t1 / K;
+            memcpy(cur, ck + (size_t)c * 19 * G.N, S);
+            for (int t = c * K; t < t1; ++t) {
+                grey_step(cur, nxt, s, G);
+                double *sw = cur; cur = nxt; nxt = sw;
+            }
+            grey_force_adj(cur, s, G, wobj, lam, gs);
+        }
+        for (int c = (t1 - 1) / K; c >= 0; --c) {
+            int tb = c * K, te = (c + 1) * K < t1 ? (c + 1) * K : t1; /* states tb..te-1 */
+            memcpy(seg, ck + (size_t)c * 19 * G.N, S);
+            for (int t = tb + 1; t < te; ++t)
+                grey_step(seg + (size_t)(t - 1 - tb) * 19 * G.N, seg + (size_t)(t - tb) * 19 * G.N, s, G);
+            for (int t = te - 1; t >= tb; --t) {
+                const double *at = seg + (size_t)(t - tb) * 19 * G.N;
+                grey_step_adj(at, s, G, lam, lam2, lin, t >= t0 ? wobj : 0.0, gs);
+                double *sw = lam; lam = lam2; lam2 = sw;
+            }
+        }
+        free(seg); free(lam); free(lam2); free(lin);
+    }
+    free(ck); free(cur); free(nxt);
+    return 0;
+}
+
+/* operator-level adjoint pieces for transpose tests */
+void orc_collide_adj(const double *f, const double *lam, double omega, double *out)
+{
+    collide_adj(f, lam, omega, out);
+}
+void orc_grey_step_adj(const double *a, const double *s, int l, int m, int n, double omega,
+                       double u_in, int xopen, const double *lam_in, double *lam_out, double wobj,
+                       double *gs)
+{
+    grey_t G = {l, m, n, omega, u_in, (long)l * m * n, xopen};
+    double *lin = (double *)malloc(sizeof(double) * 19 * G.N);
+    grey_step_adj(a, s, G, lam_in, lam_out, lin, wobj, gs);
+    free(lin);
 }

@@ -42,6 +42,11 @@ def lib():
         L.orc_probe.restype = ctypes.c_int
         L.orc_diagno.argtypes = [_dp, _ip, i, i, i, _dp]
         L.orc_mem_force.argtypes = [_dp, _ip, i, i, i, i, _dp]
+        L.orc_grey_steps.argtypes = [_dp, _dp, i, i, i, d, d, i, i]
+        L.orc_grey_force.argtypes = [_dp, _dp, i, i, i, d, i, _dp]
+        L.orc_grey_objective.argtypes = [_dp, _dp, i, i, i, d, d, i, i, i, i, d, _dp, _dp]
+        L.orc_collide_adj.argtypes = [_dp, _dp, d, _dp]
+        L.orc_grey_step_adj.argtypes = [_dp, _dp, i, i, i, d, d, i, _dp, _dp, d, _dp]
         _lib = L
     return _lib
 
@@ -157,3 +162,62 @@ def drag_boxes(lx, ly, lz, radius=100, deltas=(52, 57, 62, 67)):
         boxes.append((delta, (lx // 4 - radius - delta, lx // 4 + radius + delta,
                               ly // 2 - radius - delta, ly // 2 + radius + delta, 1, lz)))
     return boxes
+
+
+# --------------------------------------------------------------------------------------------
+# grey lattice: dense interior layout a[19, n, m, l], s[n, m, l]
+# --------------------------------------------------------------------------------------------
+
+def _dense_shape(a):
+    _, n, m, l = a.shape
+    return l, m, n
+
+
+def grey_steps(a, s, omega, u_in, nsteps, x_open=True):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    l, m, n = _dense_shape(a)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    rc = lib().orc_grey_steps(_d(a), _d(s), l, m, n, omega, u_in, int(bool(x_open)), int(nsteps))
+    assert rc == 0
+    return a
+
+
+def grey_force(a, s, u_in, x_open=True):
+    l, m, n = _dense_shape(a)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_grey_force(_d(a), _d(s), l, m, n, u_in, int(bool(x_open)), _d(out))
+    return out
+
+
+def grey_objective(a0, s, omega, u_in, t0, t1, K, norm, grad=True, x_open=True):
+    """J = mean over t0..t1 of F_x(a^t)/norm from a0; returns (J, dJ/ds or None)."""
+    a0 = np.ascontiguousarray(a0, dtype=np.float64)
+    l, m, n = _dense_shape(a0)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    J = np.zeros(1)
+    gs = np.zeros((n, m, l)) if grad else None
+    rc = lib().orc_grey_objective(_d(a0), _d(s), l, m, n, omega, u_in, int(bool(x_open)), t0, t1, K,
+                                  norm, _d(J), _d(gs) if grad else ctypes.POINTER(ctypes.c_double)())
+    assert rc == 0
+    return float(J[0]), gs
+
+
+def collide_adj(f, lam, omega):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    out = np.empty(19)
+    lib().orc_collide_adj(_d(f), _d(lam), omega, _d(out))
+    return out
+
+
+def grey_step_adj(a, s, omega, u_in, lam_in, wobj=0.0, x_open=True):
+    l, m, n = _dense_shape(a)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    lam_in = np.ascontiguousarray(lam_in, dtype=np.float64)
+    lam_out = np.empty_like(lam_in)
+    gs = np.zeros((n, m, l))
+    lib().orc_grey_step_adj(_d(a), _d(s), l, m, n, omega, u_in, int(bool(x_open)), _d(lam_in),
+                            _d(lam_out), wobj, _d(gs))
+    return lam_out, gs

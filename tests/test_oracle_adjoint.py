@@ -1,0 +1,123 @@
+"""Pins of the oracle's grey-lattice step and its discrete adjoint (no reference counterpart).
+
+* a binary solid fraction reduces the grey step to the physics step's halfway bounce-back, bit for
+  bit on fluid sites; s = 0 gives the plain fluid step;
+* partial bounce-back conserves mass on every link (periodic box);
+* the collision adjoint is the transpose of the collision Jacobian (dot-product test against
+  central differences);
+* the full adjoint gradient of the time-averaged drag matches central finite differences.
+"""
+import numpy as np
+import pytest
+
+from euplectella_hpc_amd import d3q19, geometry
+from oracle import oracle as orc
+
+
+def dense(pops):
+    return np.ascontiguousarray(pops[:, 1:-1, 1:-1, 1:-1])
+
+
+def test_binary_grey_field_is_the_physics_step_on_fluid_sites():
+    l, m, n = 24, 12, 10
+    mask = geometry.cylinder(l, m, n, 3, (8, 6))
+    pops = geometry.initial_state(l, m, n, 0.05, seed=4)
+    om = 1 / 0.56
+    ref = pops.copy()
+    orc.physics_steps(ref, mask, om, 0.05, 15)
+    a = dense(pops)
+    orc.grey_steps(a, mask.astype(np.float64), om, 0.05, 15)
+    fluid = mask == 0
+    assert np.array_equal(a[:, fluid].view(np.uint64), dense(ref)[:, fluid].view(np.uint64))
+
+
+def test_zero_grey_field_is_the_plain_fluid_step():
+    l, m, n = 16, 8, 6
+    pops = geometry.initial_state(l, m, n, 0.05, seed=5)
+    om = 1.7
+    ref = pops.copy()
+    orc.physics_steps(ref, np.zeros((n, m, l), np.uint8), om, 0.05, 10)
+    a = dense(pops)
+    orc.grey_steps(a, np.zeros((n, m, l)), om, 0.05, 10)
+    assert np.array_equal(a.view(np.uint64), dense(ref).view(np.uint64))
+
+
+def test_partial_bounce_back_conserves_mass():
+    l, m, n = 12, 10, 8
+    s = np.random.default_rng(0).random((n, m, l))
+    a = dense(geometry.initial_state(l, m, n, 0.05, noise=1e-3, seed=1))
+    m0 = a.sum()
+    orc.grey_steps(a, s, 1.6, 0.0, 30, x_open=False)
+    assert abs(a.sum() - m0) < 1e-12 * m0
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_collision_adjoint_is_the_jacobian_transpose(seed):
+    rng = np.random.default_rng(seed)
+    f = d3q19.equilibrium(1.0, 0.04, -0.02, 0.01) * (1 + 0.05 * rng.standard_normal(19))
+    lam = rng.standard_normal(19)
+    dv = rng.standard_normal(19)
+    om = 1.85
+    h = 1e-6
+    jd = (orc.collide_site(f + h * dv, om) - orc.collide_site(f - h * dv, om)) / (2 * h)
+    lhs = lam @ jd
+    rhs = orc.collide_adj(f, lam, om) @ dv
+    assert abs(lhs - rhs) < 1e-8 * max(1.0, abs(lhs))
+
+
+def test_step_adjoint_dot_product_against_central_differences():
+    """<lam, (a(t+1)(a + h d) - a(t+1)(a - h d)) / 2h> == <step_adj(lam), d>."""
+    l, m, n = 10, 6, 5
+    rng = np.random.default_rng(3)
+    s = 0.6 * rng.random((n, m, l))
+    a = dense(geometry.initial_state(l, m, n, 0.05, noise=1e-3, seed=2))
+    om, u = 1.7, 0.05
+    lam = rng.standard_normal(a.shape)
+    d = rng.standard_normal(a.shape) * 1e-3
+    h = 1e-4
+    ap = a + h * d
+    am = a - h * d
+    orc.grey_steps(ap, s, om, u, 1)
+    orc.grey_steps(am, s, om, u, 1)
+    lhs = np.sum(lam * (ap - am)) / (2 * h)
+    lam_out, _ = orc.grey_step_adj(a, s, om, u, lam)
+    rhs = np.sum(lam_out * d)
+    assert abs(lhs - rhs) < 1e-7 * abs(lhs)
+
+
+def grey_case(l=16, m=10, n=8, seed=6):
+    s = geometry.grey_spheres(l, m, n, n_spheres=3, rmin=2.0, rmax=3.0, sigma=1.0, seed=seed,
+                              region=((4, 8), (3, 7), (3, 5)))
+    a0 = dense(geometry.initial_state(l, m, n, 0.05, seed=seed))
+    return s, a0
+
+
+def test_objective_gradient_matches_central_differences():
+    s, a0 = grey_case()
+    om, u = 1 / 0.6, 0.05
+    t0, t1, K = 6, 14, 4
+    norm = 0.5 * u * u * s.shape[1] * s.shape[0]
+    J, g = orc.grey_objective(a0, s, om, u, t0, t1, K, norm)
+    rng = np.random.default_rng(1)
+    flat = rng.choice(s.size, 8, replace=False)
+    eps = 1e-5
+    for c in flat:
+        sp = s.copy().ravel()
+        sm = s.copy().ravel()
+        sp[c] += eps
+        sm[c] -= eps
+        Jp, _ = orc.grey_objective(a0, sp.reshape(s.shape), om, u, t0, t1, K, norm, grad=False)
+        Jm, _ = orc.grey_objective(a0, sm.reshape(s.shape), om, u, t0, t1, K, norm, grad=False)
+        fd = (Jp - Jm) / (2 * eps)
+        an = g.ravel()[c]
+        assert abs(fd - an) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(g).max()), (c, fd, an)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    s, a0 = grey_case(12, 8, 6, seed=2)
+    om, u = 1.6, 0.05
+    J1, g1 = orc.grey_objective(a0, s, om, u, 3, 9, 1, 1.0)
+    J2, g2 = orc.grey_objective(a0, s, om, u, 3, 9, 4, 1.0)
+    J3, g3 = orc.grey_objective(a0, s, om, u, 3, 9, 100, 1.0)
+    assert J1 == J2 == J3
+    assert np.array_equal(g1, g2) and np.array_equal(g1, g3)
