@@ -13,7 +13,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["eup_api.hip", "eup_parity.hip", "eup_physics.hip", "eup_diag.hip"]
+SOURCES = ["eup_api.hip", "eup_parity.hip", "eup_physics.hip", "eup_diag.hip", "eup_grey.hip"]
 OUT = os.path.join(HERE, "libeup.so")
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

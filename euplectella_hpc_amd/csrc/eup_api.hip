@@ -50,8 +50,12 @@ static int validate(const eup_config *c, std::string &why)
         why = "lattice too large: (l+2)(m+2)(n+2) must stay below 2^31 sites";
         return EUP_EINVAL;
     }
-    if (c->mode != EUP_MODE_PARITY && c->mode != EUP_MODE_PHYSICS) {
-        why = "mode must be EUP_MODE_PARITY or EUP_MODE_PHYSICS";
+    if (c->mode != EUP_MODE_PARITY && c->mode != EUP_MODE_PHYSICS && c->mode != EUP_MODE_GREY) {
+        why = "mode must be EUP_MODE_PARITY, EUP_MODE_PHYSICS or EUP_MODE_GREY";
+        return EUP_EINVAL;
+    }
+    if (c->mode == EUP_MODE_GREY && c->storage != EUP_STORE_F64 && c->storage != EUP_STORE_F32) {
+        why = "grey mode requires EUP_STORE_F64 or EUP_STORE_F32";
         return EUP_EINVAL;
     }
     if (storage_bytes(c->storage) == 0) {
@@ -97,7 +101,7 @@ static void release(Handle *h)
     if (h->stream) hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->buf[0], h->buf[1], h->staging, h->obs, h->solid, h->solid_list, h->bb_tmp,
                     h->tile_code, h->link_mask, h->mixed_list, h->outlet[0], h->outlet[1], h->diag,
-                    h->site19};
+                    h->site19, h->grey};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (h->ev0) hipEventDestroy(h->ev0);
@@ -159,6 +163,9 @@ int eup_create(const eup_config *cfg, eup_handle **out)
         for (int b = 0; b < 2 && !rc; ++b)
             if (hip_check(h, hipMalloc(&h->outlet[b], rb * 5 * (size_t)g.m * g.n), "hipMalloc outlet")) rc = EUP_EHIP;
     }
+    if (!rc && cfg->mode == EUP_MODE_GREY &&
+        hip_check(h, hipMemset(h->solid, 0, g.sites), "hipMemset solid"))
+        rc = EUP_EHIP;
     if (rc) {
         g_last_error = "eup_create: " + h->err;
         release(h);
@@ -181,6 +188,8 @@ void eup_destroy(eup_handle *h)
 int eup_set_mask(eup_handle *h, const int32_t *obs)
 {
     if (!h || !obs) return fail(h, EUP_EINVAL, "eup_set_mask: null argument");
+    if (h->cfg.mode == EUP_MODE_GREY)
+        return fail(h, EUP_EINVAL, "eup_set_mask: grey mode takes a solid fraction (eup_set_grey)");
     if (hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice")) return EUP_EHIP;
     const Geometry &g = h->g;
     // validate values and build the dense solid flags on the host (setup-time work)
@@ -239,6 +248,7 @@ int eup_set_pops(eup_handle *h, int32_t q, const double *a)
     int rc = h->cfg.mode == EUP_MODE_PARITY ? parity_set_pop(h, q) : physics_set_pop(h, q);
     if (rc) return rc;
     if (hip_check(h, hipStreamSynchronize(h->stream), "sync")) return EUP_EHIP;
+    h->outlet_dirty = true;
     h->pops_set |= 1u << (q - 1);
     return 0;
 }
@@ -280,7 +290,12 @@ int eup_get_state(eup_handle *h, double *a)
 static int ready(eup_handle *h, const char *who)
 {
     if (!h) return fail(nullptr, EUP_EINVAL, std::string(who) + ": null handle");
-    if (!h->mask_set) return fail(h, EUP_ESTATE, std::string(who) + ": eup_set_mask has not been called");
+    if (h->cfg.mode == EUP_MODE_GREY) {
+        if (!h->grey_set)
+            return fail(h, EUP_ESTATE, std::string(who) + ": eup_set_grey has not been called");
+    } else if (!h->mask_set) {
+        return fail(h, EUP_ESTATE, std::string(who) + ": eup_set_mask has not been called");
+    }
     if (h->pops_set != 0x7FFFFu)
         return fail(h, EUP_ESTATE, std::string(who) + ": not all 19 populations have been set");
     if (hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice")) return EUP_EHIP;
@@ -290,7 +305,9 @@ static int ready(eup_handle *h, const char *who)
 static int do_steps(eup_handle *h, int32_t nsteps)
 {
     for (int32_t t = 0; t < nsteps; ++t) {
-        int rc = h->cfg.mode == EUP_MODE_PARITY ? parity_step(h) : physics_step(h);
+        int rc = h->cfg.mode == EUP_MODE_PARITY ? parity_step(h)
+                 : h->cfg.mode == EUP_MODE_GREY ? grey_step(h)
+                                                : physics_step(h);
         if (rc) return rc;
     }
     return 0;
@@ -345,6 +362,8 @@ int eup_drag_mem(eup_handle *h, double F[3])
 {
     int rc = ready(h, "eup_drag_mem");
     if (rc) return rc;
+    if (h->cfg.mode == EUP_MODE_GREY)
+        return fail(h, EUP_EINVAL, "eup_drag_mem: grey mode uses eup_grey_force");
     if (!F) return fail(h, EUP_EINVAL, "eup_drag_mem: null argument");
     return diag_drag_mem(h, F);
 }
@@ -378,6 +397,80 @@ int eup_diagno(eup_handle *h, double out[5])
     if (rc) return rc;
     if (!out) return fail(h, EUP_EINVAL, "eup_diagno: null argument");
     return diag_diagno(h, out);
+}
+
+static int grey_ready(eup_handle *h, const char *who)
+{
+    if (!h) return fail(nullptr, EUP_EINVAL, std::string(who) + ": null handle");
+    if (h->cfg.mode != EUP_MODE_GREY) return fail(h, EUP_EINVAL, std::string(who) + ": handle is not in grey mode");
+    return hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice") ? EUP_EHIP : 0;
+}
+
+int eup_set_grey(eup_handle *h, const double *s)
+{
+    int rc = grey_ready(h, "eup_set_grey");
+    if (rc) return rc;
+    if (!s) return fail(h, EUP_EINVAL, "eup_set_grey: null argument");
+    for (int64_t x = 0; x < h->g.sites; ++x)
+        if (!(s[x] >= 0.0 && s[x] <= 1.0)) return fail(h, EUP_EINVAL, "eup_set_grey: s must lie in [0, 1]");
+    rc = grey_set_field(h, s, EUP_STORE_F64, false);
+    if (rc) return rc;
+    h->grey_set = true;
+    h->mask_set = true;
+    return 0;
+}
+
+int eup_set_grey_device(eup_handle *h, const void *s, int32_t dtype)
+{
+    int rc = grey_ready(h, "eup_set_grey_device");
+    if (rc) return rc;
+    if (!s || (dtype != EUP_STORE_F64 && dtype != EUP_STORE_F32))
+        return fail(h, EUP_EINVAL, "eup_set_grey_device: null pointer or dtype not F64/F32");
+    rc = grey_set_field(h, s, dtype, true);
+    if (rc) return rc;
+    h->grey_set = true;
+    h->mask_set = true;
+    return 0;
+}
+
+int eup_grey_force(eup_handle *h, double F[3])
+{
+    int rc = grey_ready(h, "eup_grey_force");
+    if (rc) return rc;
+    rc = ready(h, "eup_grey_force");
+    if (rc) return rc;
+    if (!F) return fail(h, EUP_EINVAL, "eup_grey_force: null argument");
+    return grey_force(h, F);
+}
+
+static int objective_args(eup_handle *h, int32_t t0, int32_t t1, int32_t every, double norm, double *J,
+                          const char *who)
+{
+    int rc = grey_ready(h, who);
+    if (rc) return rc;
+    rc = ready(h, who);
+    if (rc) return rc;
+    if (!J || t0 < 1 || t1 < t0 || every < 1 || !(norm != 0.0))
+        return fail(h, EUP_EINVAL, std::string(who) + ": need J != NULL, 1 <= t0 <= t1, every >= 1, norm != 0");
+    return 0;
+}
+
+int eup_grey_objective(eup_handle *h, int32_t t0, int32_t t1, int32_t every, double norm, double *J,
+                       double *grad)
+{
+    int rc = objective_args(h, t0, t1, every, norm, J, "eup_grey_objective");
+    if (rc) return rc;
+    return grey_objective(h, t0, t1, every, norm, J, nullptr, EUP_STORE_F64, grad);
+}
+
+int eup_grey_objective_device(eup_handle *h, int32_t t0, int32_t t1, int32_t every, double norm,
+                              double *J, void *grad, int32_t dtype)
+{
+    int rc = objective_args(h, t0, t1, every, norm, J, "eup_grey_objective_device");
+    if (rc) return rc;
+    if (grad && dtype != EUP_STORE_F64 && dtype != EUP_STORE_F32)
+        return fail(h, EUP_EINVAL, "eup_grey_objective_device: dtype must be F64 or F32");
+    return grey_objective(h, t0, t1, every, norm, J, grad, dtype, nullptr);
 }
 
 int eup_get_info(eup_handle *h, eup_info *info)

@@ -1,0 +1,621 @@
+// Grey-lattice mode: partial bounce-back on a solid-fraction field s in [0, 1], the drag
+// objective averaged over a step window, and its discrete adjoint with checkpoint-recompute.
+// (No reference counterpart; the binary-mask limit is the physics step's bounce-back.)
+//
+// Link model: the link between x and its source y = x - c_q has the fraction
+//   sigma = 1 - (1 - s(x))(1 - s(y))          (s = 0 on the inlet / outlet ghosts)
+// and the pulled population is f_q(x) = (1 - sigma) src_q + sigma a_qbar(x), with src_q = a_q(y),
+// the inlet equilibrium or the outlet equilibrium of bcond.bc.F90.  sigma is symmetric in the
+// link, so every link conserves mass; s in {0, 1} gives the halfway bounce-back exactly.
+// Force on the grey body: F = sum over interior links (x, q) of sigma c_q (a_q(y) - a_qbar(x)).
+// Objective: J = (1 / ((t1 - t0 + 1) norm)) sum_{t = t0..t1} F_x(a^t).
+//
+// Adjoint: lam^t = (d a^{t+1} / d a^t)^T lam^{t+1} + [t >= t0] w dF_x/da(a^t), and
+// dJ/ds accumulates (d a^{t+1} / ds)^T lam^{t+1} + [t >= t0] w dF_x/ds(a^t).  Per step two kernels:
+// k_adj_collide recomputes the pulled populations f(x) of a^t and applies the transposed BGK
+// Jacobian to lam^{t+1}; k_adj_stream gathers the transposed pull (all gathers, no atomics):
+//   lam_p(z) = (1 - sigma_l) lin_p(z + c_p) + sigma_l lin_pbar(z)   (sigma_l of link z -- z + c_p)
+// plus the outlet-equilibrium transpose on the last x column and the objective terms, and adds the
+// site's gradient contributions from both ends of each of its links (FP64 accumulation).
+// States are checkpointed every K steps; each segment is recomputed forward and stored before its
+// backward sweep, so memory is (t1/K + K) states instead of t1.
+#include <type_traits>
+#include <vector>
+
+#include "eup_internal.h"
+
+namespace eup {
+
+namespace {
+
+template <typename R> struct GArgs {
+    const R *A[19];
+    R *B[19];
+    const R *s;
+    R inl[5];
+    R omega;
+    int l, m, n;
+    int xopen;
+};
+
+struct Nbr {
+    int l, m, n, xopen;
+    // site index of (i + dx, j + dy, k + dz), wrapping y and z (and x when periodic); -1 = inlet
+    // ghost, -2 = outlet ghost
+    __device__ __forceinline__ int64_t at(int i, int j, int k, int dx, int dy, int dz) const
+    {
+        int ii = i + dx, jj = j + dy, kk = k + dz;
+        jj = jj < 0 ? jj + m : (jj >= m ? jj - m : jj);
+        kk = kk < 0 ? kk + n : (kk >= n ? kk - n : kk);
+        if (ii < 0) {
+            if (xopen) return -1;
+            ii += l;
+        } else if (ii >= l) {
+            if (xopen) return -2;
+            ii -= l;
+        }
+        return (int64_t)ii + (int64_t)l * ((int64_t)jj + (int64_t)m * kk);
+    }
+};
+
+__device__ __constant__ int8_t dCX[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+__device__ __constant__ int8_t dCY[19] = {-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1, 0};
+__device__ __constant__ int8_t dCZ[19] = {0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1, 0};
+__device__ __constant__ int8_t dOPP[19] = {11, 12, 9, 10, 13, 14, 15, 16, 17, 2, 3, 0, 1, 4, 5, 6, 7, 8, 18};
+
+template <typename R> __device__ __forceinline__ R wq(int q)
+{
+    return q == 18 ? R(1) / R(3) : ((dCX[q] * dCX[q] + dCY[q] * dCY[q] + dCZ[q] * dCZ[q]) == 1 ? R(1) / R(18) : R(1) / R(36));
+}
+
+// pulled populations f[19] of site (i,j,k) (0-based); sig[q], src[q] and kind[q]
+// (0 interior, 1 inlet, 2 outlet) describe the links, ysite[q] the source site
+template <typename R>
+__device__ void grey_pull(const GArgs<R> &a, const Nbr &nb, int i, int j, int k, int64_t x, R *f,
+                          R *sig, R *src, int *kind, int64_t *ysite)
+{
+    const R sx = a.s[x];
+    for (int q = 0; q < 18; ++q) {
+        const int64_t y = nb.at(i, j, k, -dCX[q], -dCY[q], -dCZ[q]);
+        R sv, sy;
+        if (y == -1) {
+            sv = a.inl[q];
+            sy = R(0);
+            kind[q] = 1;
+            ysite[q] = -1;
+        } else if (y == -2) {
+            const int64_t z = nb.at(a.l - 1, j, k, 0, -dCY[q], -dCZ[q]);
+            R yv[19], o[5];
+            for (int p = 0; p < 19; ++p) yv[p] = a.A[p][z];
+            outlet_eq<R>(yv, o);
+            sv = o[q - 9];
+            sy = R(0);
+            kind[q] = 2;
+            ysite[q] = z;
+        } else {
+            sv = a.A[q][y];
+            sy = a.s[y];
+            kind[q] = 0;
+            ysite[q] = y;
+        }
+        const R sg = R(1) - (R(1) - sx) * (R(1) - sy);
+        sig[q] = sg;
+        src[q] = sv;
+        f[q] = (R(1) - sg) * sv + sg * a.A[dOPP[q]][x];
+    }
+    f[18] = a.A[18][x];
+}
+
+template <typename R> __global__ __launch_bounds__(256) void k_grey_step(GArgs<R> a)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = (int64_t)a.l * a.m * a.n;
+    if (x >= N) return;
+    const int i = (int)(x % a.l);
+    const int64_t r = x / a.l;
+    const int j = (int)(r % a.m), k = (int)(r / a.m);
+    Nbr nb{a.l, a.m, a.n, a.xopen};
+    R f[19], sig[18], src[18];
+    int kind[18];
+    int64_t ys[18];
+    grey_pull<R>(a, nb, i, j, k, x, f, sig, src, kind, ys);
+    collide<R>(f, f, a.omega, R(0));
+    for (int q = 0; q < 19; ++q) a.B[q][x] = f[q];
+}
+
+__device__ __forceinline__ double wsum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// block partial sums of the force (3 doubles per block)
+template <typename R> __global__ __launch_bounds__(256) void k_grey_force(GArgs<R> a, double *partial)
+{
+    const int64_t N = (int64_t)a.l * a.m * a.n;
+    Nbr nb{a.l, a.m, a.n, a.xopen};
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(x % a.l);
+        const int64_t r = x / a.l;
+        const int j = (int)(r % a.m), k = (int)(r / a.m);
+        R f[19], sig[18], src[18];
+        int kind[18];
+        int64_t ys[18];
+        grey_pull<R>(a, nb, i, j, k, x, f, sig, src, kind, ys);
+        for (int q = 0; q < 18; ++q) {
+            if (kind[q] != 0) continue;
+            const double d = (double)sig[q] * ((double)src[q] - (double)a.A[dOPP[q]][x]);
+            v[0] += dCX[q] * d;
+            v[1] += dCY[q] * d;
+            v[2] += dCZ[q] * d;
+        }
+    }
+    __shared__ double sh[3][4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int c = 0; c < 3; ++c) {
+        double t = wsum(v[c]);
+        if (lane == 0) sh[c][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) partial[blockIdx.x * 3 + c] = sh[c][0] + sh[c][1] + sh[c][2] + sh[c][3];
+}
+
+// acc[c] += weight * sum of partials (single block, fixed order)
+__global__ void k_force_acc(const double *partial, int nb, double weight, double *acc, double *last)
+{
+    __shared__ double sh[3][256];
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int c = 0; c < 3; ++c) v[c] += partial[b * 3 + c];
+    for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) {
+            last[c] = sh[c][0];
+            acc[c] += weight * sh[c][0];
+        }
+}
+
+// transposed BGK Jacobian at pre-collision f applied to lam
+template <typename R> __device__ void collide_adj(const R *f, const R *lam, R omega, R *out)
+{
+    R rho = 0, jx = 0, jy = 0, jz = 0;
+    for (int q = 0; q < 19; ++q) {
+        rho += f[q];
+        jx += dCX[q] * f[q];
+        jy += dCY[q] * f[q];
+        jz += dCZ[q] * f[q];
+    }
+    const R ux = jx / rho, uy = jy / rho, uz = jz / rho, usq = ux * ux + uy * uy + uz * uz;
+    R A = 0, Bx = 0, By = 0, Bz = 0;
+    for (int q = 0; q < 19; ++q) {
+        const R w = wq<R>(q);
+        const R cu = dCX[q] * ux + dCY[q] * uy + dCZ[q] * uz;
+        A += lam[q] * w * (R(1) - R(4.5) * cu * cu + R(1.5) * usq);
+        const R g = lam[q] * w * (R(3) + R(9) * cu);
+        Bx += g * dCX[q] - lam[q] * w * R(3) * ux;
+        By += g * dCY[q] - lam[q] * w * R(3) * uy;
+        Bz += g * dCZ[q] - lam[q] * w * R(3) * uz;
+    }
+    for (int p = 0; p < 19; ++p)
+        out[p] = (R(1) - omega) * lam[p] + omega * (A + Bx * dCX[p] + By * dCY[p] + Bz * dCZ[p]);
+}
+
+template <typename R> struct AdjArgs {
+    GArgs<R> g;
+    const R *lam_in[19];  // lam^{t+1}
+    R *lin[19];           // J_C^T lam^{t+1} at every site
+    R *lam_out[19];       // lam^t
+    double *gs;
+    double wobj;                       // objective weight at time t (0 outside the window)
+};
+
+template <typename R> __global__ __launch_bounds__(256) void k_adj_collide(AdjArgs<R> a)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = (int64_t)a.g.l * a.g.m * a.g.n;
+    if (x >= N) return;
+    const int i = (int)(x % a.g.l);
+    const int64_t r = x / a.g.l;
+    const int j = (int)(r % a.g.m), k = (int)(r / a.g.m);
+    Nbr nb{a.g.l, a.g.m, a.g.n, a.g.xopen};
+    R f[19], sig[18], src[18], lam[19], out[19];
+    int kind[18];
+    int64_t ys[18];
+    grey_pull<R>(a.g, nb, i, j, k, x, f, sig, src, kind, ys);
+    for (int q = 0; q < 19; ++q) lam[q] = a.lam_in[q][x];
+    collide_adj<R>(f, lam, a.g.omega, out);
+    for (int q = 0; q < 19; ++q) a.lin[q][x] = out[q];
+}
+
+template <typename R> __global__ __launch_bounds__(256) void k_adj_stream(AdjArgs<R> a)
+{
+    const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = a.g.l, m = a.g.m, n = a.g.n;
+    const int64_t N = (int64_t)l * m * n;
+    if (z >= N) return;
+    const int i = (int)(z % l);
+    const int64_t r = z / l;
+    const int j = (int)(r % m), k = (int)(r / m);
+    Nbr nb{l, m, n, a.g.xopen};
+    const double w = a.wobj;
+    const R sz = a.g.s[z];
+    double g = 0.0;
+    // (A) z as the pulling site x
+    {
+        R f[19], sig[18], src[18];
+        int kind[18];
+        int64_t ys[18];
+        grey_pull<R>(a.g, nb, i, j, k, z, f, sig, src, kind, ys);
+        for (int q = 0; q < 18; ++q) {
+            const double L = (double)a.lin[q][z];
+            const double abar = (double)a.g.A[dOPP[q]][z];
+            const double sy = kind[q] == 0 ? (double)a.g.s[ys[q]] : 0.0;
+            g += L * (abar - (double)src[q]) * (1.0 - sy);
+            if (kind[q] == 0 && w != 0.0) g += w * dCX[q] * ((double)src[q] - abar) * (1.0 - sy);
+        }
+    }
+    for (int p = 0; p < 18; ++p) {
+        const int pb = dOPP[p];
+        const int64_t zp = nb.at(i, j, k, dCX[p], dCY[p], dCZ[p]); // the site pulling p from z
+        double lam;
+        if (zp >= 0) {
+            const R s2 = a.g.s[zp];
+            const double sl = 1.0 - (1.0 - (double)sz) * (1.0 - (double)s2);
+            lam = (1.0 - sl) * (double)a.lin[p][zp] + sl * (double)a.lin[pb][z];
+            if (w != 0.0) lam += 2.0 * w * sl * dCX[p];
+            // (B) z as the source y of the pair (zp, p)
+            const double d = (double)a.g.A[pb][zp] - (double)a.g.A[p][z];
+            g += (double)a.lin[p][zp] * d * (1.0 - (double)s2);
+            if (w != 0.0) g -= w * dCX[p] * d * (1.0 - (double)s2);
+        } else {
+            lam = (double)sz * (double)a.lin[pb][z]; // ghost link: sigma = s(z)
+        }
+        a.lam_out[p][z] = (R)lam;
+    }
+    a.lam_out[18][z] = a.lin[18][z];
+    // outlet equilibrium transpose: z on the last x column feeds the ghost values read by the
+    // sites x = (l-1, j + cy, k + cz) for q = 10..14 (0-based 9..13)
+    if (a.g.xopen && i == l - 1) {
+        R y[19];
+        for (int p = 0; p < 19; ++p) y[p] = a.g.A[p][z];
+        double Jx = 0, Jy = 0, Jz = 0;
+        for (int p = 0; p < 19; ++p) {
+            Jx += dCX[p] * (double)y[p];
+            Jy += dCY[p] * (double)y[p];
+            Jz += dCZ[p] * (double)y[p];
+        }
+        double acc[19];
+        for (int p = 0; p < 19; ++p) acc[p] = 0.0;
+        for (int q = 9; q < 14; ++q) {
+            const int64_t x = nb.at(l - 1, j, k, 0, dCY[q], dCZ[q]);
+            const double sg = (double)a.g.s[x]; // sigma of the ghost link
+            const double L = (1.0 - sg) * (double)a.lin[q][x];
+            const double wv = (double)wq<double>(q);
+            const double cJ = dCX[q] * Jx + dCY[q] * Jy + dCZ[q] * Jz;
+            for (int p = 0; p < 19; ++p) {
+                const double cc = dCX[q] * dCX[p] + dCY[q] * dCY[p] + dCZ[q] * dCZ[p];
+                const double Jc = Jx * dCX[p] + Jy * dCY[p] + Jz * dCZ[p];
+                acc[p] += L * wv * (3.0 * cc + 9.0 * cJ * cc - 3.0 * Jc);
+            }
+        }
+        for (int p = 0; p < 19; ++p) a.lam_out[p][z] = (R)((double)a.lam_out[p][z] + acc[p]);
+    }
+    a.gs[z] += g;
+}
+
+// objective seed at t1: lam = w dF/da, gs += w dF/ds
+template <typename R> __global__ __launch_bounds__(256) void k_force_adj_seed(AdjArgs<R> a)
+{
+    const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = a.g.l, m = a.g.m, n = a.g.n;
+    const int64_t N = (int64_t)l * m * n;
+    if (z >= N) return;
+    const int i = (int)(z % l);
+    const int64_t r = z / l;
+    const int j = (int)(r % m), k = (int)(r / m);
+    Nbr nb{l, m, n, a.g.xopen};
+    const double w = a.wobj;
+    const double sz = (double)a.g.s[z];
+    double g = 0.0;
+    for (int q = 0; q < 18; ++q) { // z as x: pair (z, q) with interior source y = z - c_q
+        const int64_t y = nb.at(i, j, k, -dCX[q], -dCY[q], -dCZ[q]);
+        if (y < 0) continue;
+        const double d = (double)a.g.A[q][y] - (double)a.g.A[dOPP[q]][z];
+        g += w * dCX[q] * d * (1.0 - (double)a.g.s[y]);
+    }
+    for (int p = 0; p < 18; ++p) {
+        const int64_t zp = nb.at(i, j, k, dCX[p], dCY[p], dCZ[p]);
+        double lam = 0.0;
+        if (zp >= 0) {
+            const double s2 = (double)a.g.s[zp];
+            const double sl = 1.0 - (1.0 - sz) * (1.0 - s2);
+            lam = 2.0 * w * sl * dCX[p];
+            const double d = (double)a.g.A[p][z] - (double)a.g.A[dOPP[p]][zp];
+            g += w * dCX[p] * d * (1.0 - s2);
+        }
+        a.lam_out[p][z] = (R)lam;
+    }
+    a.lam_out[18][z] = R(0);
+    a.gs[z] += g;
+}
+
+template <typename R> GArgs<R> gargs(Handle *h, const void *from, void *to)
+{
+    GArgs<R> a;
+    const R *src = static_cast<const R *>(from);
+    R *dst = static_cast<R *>(to);
+    for (int q = 0; q < 19; ++q) {
+        a.A[q] = src + (int64_t)q * h->stride;
+        a.B[q] = dst ? dst + (int64_t)q * h->stride : nullptr;
+    }
+    a.s = static_cast<const R *>(h->grey);
+    R inl[5];
+    inlet_eq<R>((R)h->cfg.u_inflow, inl);
+    for (int u = 0; u < 5; ++u) a.inl[u] = inl[u];
+    a.omega = (R)h->cfg.omega;
+    a.l = h->g.l;
+    a.m = h->g.m;
+    a.n = h->g.n;
+    a.xopen = h->cfg.front != EUP_FACE_PERIODIC;
+    return a;
+}
+
+template <typename R> unsigned grid_sites(Handle *h) { return (unsigned)((h->g.sites + 255) / 256); }
+
+constexpr int kForceBlocks = 1024;
+
+template <typename R> int grey_step_t(Handle *h)
+{
+    GArgs<R> a = gargs<R>(h, h->buf[h->cur], h->buf[h->cur ^ 1]);
+    hipLaunchKernelGGL((k_grey_step<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, a);
+    if (hip_check(h, hipGetLastError(), "grey step")) return EUP_EHIP;
+    h->cur ^= 1;
+    return 0;
+}
+
+template <typename R> int force_into(Handle *h, const void *state, double weight, double *acc, double *last)
+{
+    GArgs<R> a = gargs<R>(h, state, nullptr);
+    hipLaunchKernelGGL((k_grey_force<R>), dim3(kForceBlocks), dim3(256), 0, h->stream, a, h->diag);
+    hipLaunchKernelGGL(k_force_acc, dim3(1), dim3(256), 0, h->stream, h->diag, kForceBlocks, weight, acc, last);
+    return hip_check(h, hipGetLastError(), "grey force") ? EUP_EHIP : 0;
+}
+
+template <typename R>
+int objective_t(Handle *h, int t0, int t1, int K, double norm, double *J, void *grad_dev, int gdtype,
+                double *grad_host)
+{
+    const int64_t N = h->g.sites;
+    const size_t S = sizeof(R) * (size_t)h->stride * 19; // one state
+    const int nck = t1 / K + 1;
+    const bool want_grad = grad_dev || grad_host;
+    std::vector<void *> mem;
+    auto alloc = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        mem.push_back(p);
+        return p;
+    };
+    auto cleanup = [&]() {
+        for (void *p : mem) hipFree(p);
+    };
+    if (h->diag_len < 3 * kForceBlocks + 16) {
+        hipFree(h->diag);
+        h->diag = nullptr;
+        h->diag_len = 0;
+        if (hip_check(h, hipMalloc((void **)&h->diag, sizeof(double) * (3 * kForceBlocks + 16)), "hipMalloc"))
+            return EUP_EHIP;
+        h->diag_len = 3 * kForceBlocks + 16;
+    }
+    double *acc = h->diag + 3 * kForceBlocks;     // [0..2] accumulator, [3..5] last force
+    char *ck = static_cast<char *>(alloc(S * nck));
+    void *work[2] = {alloc(S), alloc(S)};
+    if (!ck || !work[0] || !work[1]) {
+        cleanup();
+        return fail(h, EUP_EHIP, "eup_grey_objective: out of device memory for checkpoints");
+    }
+    if (hip_check(h, hipMemsetAsync(acc, 0, sizeof(double) * 6, h->stream), "memset")) {
+        cleanup();
+        return EUP_EHIP;
+    }
+    // forward sweep from the current state
+    void *init = h->buf[h->cur];
+    if (hip_check(h, hipMemcpyAsync(work[0], init, S, hipMemcpyDeviceToDevice, h->stream), "copy")) {
+        cleanup();
+        return EUP_EHIP;
+    }
+    int c = 0;
+    for (int t = 0; t <= t1; ++t) {
+        void *cur = work[c];
+        if (t % K == 0)
+            hipMemcpyAsync(ck + S * (size_t)(t / K), cur, S, hipMemcpyDeviceToDevice, h->stream);
+        if (t >= t0 && force_into<R>(h, cur, 1.0, acc, acc + 3)) {
+            cleanup();
+            return EUP_EHIP;
+        }
+        if (t == t1) break;
+        GArgs<R> a = gargs<R>(h, cur, work[c ^ 1]);
+        hipLaunchKernelGGL((k_grey_step<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, a);
+        c ^= 1;
+    }
+    double sum[3];
+    if (hip_check(h, hipGetLastError(), "forward sweep") ||
+        hip_check(h, hipMemcpyAsync(sum, acc, sizeof(double) * 3, hipMemcpyDeviceToHost, h->stream), "copy") ||
+        hip_check(h, hipStreamSynchronize(h->stream), "sync")) {
+        cleanup();
+        return EUP_EHIP;
+    }
+    const double wobj = 1.0 / ((double)(t1 - t0 + 1) * norm);
+    *J = sum[0] * wobj;
+    if (want_grad) {
+        char *seg = static_cast<char *>(alloc(S * (size_t)K));
+        void *lamb[2] = {alloc(S), alloc(S)};
+        void *lin = alloc(S);
+        double *gs = static_cast<double *>(alloc(sizeof(double) * N));
+        if (!seg || !lamb[0] || !lamb[1] || !lin || !gs) {
+            cleanup();
+            return fail(h, EUP_EHIP, "eup_grey_objective: out of device memory for the adjoint");
+        }
+        hipMemsetAsync(gs, 0, sizeof(double) * N, h->stream);
+        auto adj = [&](const void *state, int li, int lo, double w) {
+            AdjArgs<R> q;
+            q.g = gargs<R>(h, state, nullptr);
+            for (int p = 0; p < 19; ++p) {
+                q.lam_in[p] = static_cast<const R *>(lamb[li]) + (int64_t)p * h->stride;
+                q.lin[p] = static_cast<R *>(lin) + (int64_t)p * h->stride;
+                q.lam_out[p] = static_cast<R *>(lamb[lo]) + (int64_t)p * h->stride;
+            }
+            q.gs = gs;
+            q.wobj = w;
+            return q;
+        };
+        // seed at t1: recompute a^{t1} from its checkpoint
+        {
+            const int cc = t1 / K;
+            hipMemcpyAsync(work[0], ck + S * (size_t)cc, S, hipMemcpyDeviceToDevice, h->stream);
+            int w = 0;
+            for (int t = cc * K; t < t1; ++t) {
+                GArgs<R> a = gargs<R>(h, work[w], work[w ^ 1]);
+                hipLaunchKernelGGL((k_grey_step<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, a);
+                w ^= 1;
+            }
+            AdjArgs<R> q = adj(work[w], 1, 0, wobj);
+            hipLaunchKernelGGL((k_force_adj_seed<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, q);
+        }
+        int li = 0; // lamb[li] holds lam^{t+1}
+        for (int cc = (t1 - 1) / K; cc >= 0; --cc) {
+            const int tb = cc * K, te = (cc + 1) * K < t1 ? (cc + 1) * K : t1;
+            hipMemcpyAsync(seg, ck + S * (size_t)cc, S, hipMemcpyDeviceToDevice, h->stream);
+            for (int t = tb + 1; t < te; ++t) {
+                GArgs<R> a = gargs<R>(h, seg + S * (size_t)(t - 1 - tb), seg + S * (size_t)(t - tb));
+                hipLaunchKernelGGL((k_grey_step<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, a);
+            }
+            for (int t = te - 1; t >= tb; --t) {
+                AdjArgs<R> q = adj(seg + S * (size_t)(t - tb), li, li ^ 1, t >= t0 ? wobj : 0.0);
+                hipLaunchKernelGGL((k_adj_collide<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, q);
+                hipLaunchKernelGGL((k_adj_stream<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, q);
+                li ^= 1;
+            }
+        }
+        if (hip_check(h, hipGetLastError(), "adjoint sweep")) {
+            cleanup();
+            return EUP_EHIP;
+        }
+        int rc = 0;
+        if (grad_host) {
+            if (hip_check(h, hipMemcpyAsync(grad_host, gs, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream), "copy grad"))
+                rc = EUP_EHIP;
+        } else if (gdtype == EUP_STORE_F64) {
+            if (hip_check(h, hipMemcpyAsync(grad_dev, gs, sizeof(double) * N, hipMemcpyDeviceToDevice, h->stream), "copy grad"))
+                rc = EUP_EHIP;
+        } else {
+            rc = 0;
+            float *gd = static_cast<float *>(grad_dev);
+            std::vector<double> tmp(N);
+            if (hip_check(h, hipMemcpyAsync(tmp.data(), gs, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream), "copy") ||
+                hip_check(h, hipStreamSynchronize(h->stream), "sync"))
+                rc = EUP_EHIP;
+            std::vector<float> tf(N);
+            for (int64_t x = 0; x < N; ++x) tf[x] = (float)tmp[x];
+            if (!rc && hip_check(h, hipMemcpy(gd, tf.data(), sizeof(float) * N, hipMemcpyHostToDevice), "copy"))
+                rc = EUP_EHIP;
+        }
+        if (!rc && hip_check(h, hipStreamSynchronize(h->stream), "sync")) rc = EUP_EHIP;
+        if (rc) {
+            cleanup();
+            return rc;
+        }
+    }
+    cleanup();
+    return 0;
+}
+
+template <typename F> int rdispatch(Handle *h, F &&f)
+{
+    if (h->cfg.storage == EUP_STORE_F64) return f((double *)nullptr);
+    return f((float *)nullptr);
+}
+
+template <typename R>
+__global__ void k_grey_from(const void *src, int dtype, R *dst, int64_t N)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= N) return;
+    double v = dtype == EUP_STORE_F64 ? static_cast<const double *>(src)[x] : (double)static_cast<const float *>(src)[x];
+    dst[x] = (R)v;
+}
+
+} // namespace
+
+int grey_step(Handle *h)
+{
+    return rdispatch(h, [&](auto *p) { return grey_step_t<std::remove_pointer_t<decltype(p)>>(h); });
+}
+
+int grey_set_field(Handle *h, const void *s, int dtype, bool on_device)
+{
+    const int64_t N = h->g.sites;
+    const size_t rb = h->cfg.storage == EUP_STORE_F64 ? 8 : 4;
+    if (!h->grey && hip_check(h, hipMalloc(&h->grey, rb * N), "hipMalloc grey")) return EUP_EHIP;
+    const void *src = s;
+    void *tmp = nullptr;
+    if (!on_device) {
+        if (hip_check(h, hipMalloc(&tmp, sizeof(double) * N), "hipMalloc") ||
+            hip_check(h, hipMemcpy(tmp, s, sizeof(double) * N, hipMemcpyHostToDevice), "copy grey")) {
+            hipFree(tmp);
+            return EUP_EHIP;
+        }
+        src = tmp;
+        dtype = EUP_STORE_F64;
+    }
+    int rc = rdispatch(h, [&](auto *p) {
+        using R = std::remove_pointer_t<decltype(p)>;
+        hipLaunchKernelGGL((k_grey_from<R>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, src,
+                           dtype, static_cast<R *>(h->grey), N);
+        return hip_check(h, hipGetLastError(), "grey field") ? EUP_EHIP : 0;
+    });
+    if (!rc && hip_check(h, hipStreamSynchronize(h->stream), "sync")) rc = EUP_EHIP;
+    hipFree(tmp);
+    return rc;
+}
+
+int grey_force(Handle *h, double F[3])
+{
+    if (h->diag_len < 3 * kForceBlocks + 16) {
+        hipFree(h->diag);
+        h->diag = nullptr;
+        h->diag_len = 0;
+        if (hip_check(h, hipMalloc((void **)&h->diag, sizeof(double) * (3 * kForceBlocks + 16)), "hipMalloc"))
+            return EUP_EHIP;
+        h->diag_len = 3 * kForceBlocks + 16;
+    }
+    double *acc = h->diag + 3 * kForceBlocks;
+    if (hip_check(h, hipMemsetAsync(acc, 0, sizeof(double) * 6, h->stream), "memset")) return EUP_EHIP;
+    int rc = rdispatch(h, [&](auto *p) {
+        return force_into<std::remove_pointer_t<decltype(p)>>(h, h->buf[h->cur], 1.0, acc, acc + 3);
+    });
+    if (rc) return rc;
+    if (hip_check(h, hipMemcpyAsync(F, acc + 3, sizeof(double) * 3, hipMemcpyDeviceToHost, h->stream), "copy") ||
+        hip_check(h, hipStreamSynchronize(h->stream), "sync"))
+        return EUP_EHIP;
+    return 0;
+}
+
+int grey_objective(Handle *h, int t0, int t1, int K, double norm, double *J, void *grad_dev, int gdtype,
+                   double *grad_host)
+{
+    return rdispatch(h, [&](auto *p) {
+        return objective_t<std::remove_pointer_t<decltype(p)>>(h, t0, t1, K, norm, J, grad_dev, gdtype,
+                                                               grad_host);
+    });
+}
+
+} // namespace eup

@@ -58,6 +58,9 @@ struct Handle {
     double *diag = nullptr;       // device scratch for reductions
     int64_t diag_len = 0;
     double *site19 = nullptr;     // one site's 19 populations in FP64
+    // grey mode
+    void *grey = nullptr;         // solid fraction (compute type), l*m*n
+    bool grey_set = false;
     int64_t device_bytes = 0;
 };
 
@@ -76,6 +79,13 @@ int physics_set_pop(Handle *h, int q); // staging (ghosted f64) -> both buffers
 int physics_get_pop(Handle *h, int q); // current buffer -> staging
 int parity_set_pop(Handle *h, int q);
 int parity_get_pop(Handle *h, int q);
+
+// grey mode (eup_grey.hip)
+int grey_step(Handle *h);
+int grey_set_field(Handle *h, const void *s, int dtype, bool on_device);
+int grey_force(Handle *h, double F[3]);
+int grey_objective(Handle *h, int t0, int t1, int K, double norm, double *J, void *grad_dev, int gdtype,
+                   double *grad_host);
 
 // diagnostics (eup_diag.hip)
 int diag_site(Handle *h, int i, int j, int k, double out[19]);

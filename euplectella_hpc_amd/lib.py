@@ -19,7 +19,8 @@ ERRORS = {-1: "EUP_EINVAL", -2: "EUP_EHIP", -3: "EUP_ESTATE", -4: "EUP_ERANGE"}
 
 STORE_F64, STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2, 3
 STORAGE = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
-MODE_PARITY, MODE_PHYSICS = 0, 1
+MODE_PARITY, MODE_PHYSICS, MODE_GREY = 0, 1, 2
+MODES = {"parity": MODE_PARITY, "physics": MODE_PHYSICS, "grey": MODE_GREY}
 FACE_PERIODIC, FACE_INFLOW, FACE_OUTFLOW = 0, 4, 5
 FLAG_NO_TILE_SKIP = 1
 
@@ -27,7 +28,8 @@ FLAG_NO_TILE_SKIP = 1
 EXPORTS = ("eup_create", "eup_destroy", "eup_last_error", "eup_set_mask", "eup_set_pops",
            "eup_get_pops", "eup_set_state", "eup_get_state", "eup_step", "eup_step_timed",
            "eup_sync", "eup_drag_cv", "eup_drag_mem", "eup_probe", "eup_diagno", "eup_get_info",
-           "eup_stream", "eup_abi_version")
+           "eup_stream", "eup_abi_version", "eup_set_grey", "eup_set_grey_device",
+           "eup_grey_force", "eup_grey_objective", "eup_grey_objective_device")
 
 
 class EupConfig(ctypes.Structure):
@@ -90,6 +92,11 @@ def load():
     L.eup_stream.restype = vp
     L.eup_abi_version.argtypes = []
     L.eup_abi_version.restype = i32
+    L.eup_set_grey.argtypes = [vp, dp]
+    L.eup_set_grey_device.argtypes = [vp, vp, i32]
+    L.eup_grey_force.argtypes = [vp, dp]
+    L.eup_grey_objective.argtypes = [vp, i32, i32, i32, ctypes.c_double, dp, dp]
+    L.eup_grey_objective_device.argtypes = [vp, i32, i32, i32, ctypes.c_double, dp, vp, i32]
     _lib = L
     return L
 
@@ -125,7 +132,7 @@ class Lattice:
         self.u_inflow = float(u_inflow)
         cfg = EupConfig()
         cfg.l, cfg.m, cfg.n = l, m, n
-        cfg.mode = MODE_PARITY if mode == "parity" else MODE_PHYSICS
+        cfg.mode = MODES[mode]
         cfg.storage = STORAGE[storage]
         cfg.front = FACE_OUTFLOW if x_open else FACE_PERIODIC
         cfg.rear = FACE_INFLOW if x_open else FACE_PERIODIC
@@ -234,6 +241,41 @@ class Lattice:
         out = np.zeros(5)
         self._check(self.L.eup_diagno(self.h, _dptr(out)))
         return out
+
+    # -- grey lattice -----------------------------------------------------------------------------
+    def set_grey(self, s):
+        """Solid fraction s[k, j, i] in [0, 1] (grey mode)."""
+        l, m, n = self.shape
+        a = np.ascontiguousarray(s, dtype=np.float64)
+        if a.shape != (n, m, l):
+            raise ValueError(f"grey field shape {a.shape} != {(n, m, l)}")
+        self._check(self.L.eup_set_grey(self.h, _dptr(a)))
+
+    def set_grey_device(self, ptr, dtype):
+        """Solid fraction from device memory (``dtype`` 'f64' or 'f32')."""
+        self._check(self.L.eup_set_grey_device(self.h, ctypes.c_void_p(ptr), STORAGE[dtype]))
+
+    def grey_force(self):
+        F = np.zeros(3)
+        self._check(self.L.eup_grey_force(self.h, _dptr(F)))
+        return F
+
+    def grey_objective(self, t0, t1, every, norm, grad=True):
+        """(J, dJ/ds) of the time-averaged drag F_x / norm over steps t0..t1 (state unchanged)."""
+        l, m, n = self.shape
+        J = ctypes.c_double()
+        g = np.zeros((n, m, l)) if grad else None
+        self._check(self.L.eup_grey_objective(self.h, int(t0), int(t1), int(every), float(norm),
+                                              ctypes.byref(J),
+                                              _dptr(g) if grad else ctypes.POINTER(ctypes.c_double)()))
+        return J.value, g
+
+    def grey_objective_device(self, t0, t1, every, norm, grad_ptr=None, dtype="f64"):
+        J = ctypes.c_double()
+        self._check(self.L.eup_grey_objective_device(
+            self.h, int(t0), int(t1), int(every), float(norm), ctypes.byref(J),
+            ctypes.c_void_p(grad_ptr) if grad_ptr else None, STORAGE[dtype]))
+        return J.value
 
     def info(self):
         inf = EupInfo()

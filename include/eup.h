@@ -49,6 +49,9 @@ extern "C" {
 #define EUP_MODE_PARITY 0  /* the reference's serial step, bit for bit (FP64, ghosted arrays) */
 #define EUP_MODE_PHYSICS 1 /* fused pull collide+stream: y/z periodic by index wrap, bounce-back
                               from the site's own opposite population, solid sites frozen */
+#define EUP_MODE_GREY 2    /* physics semantics on a solid-fraction field s in [0,1] (partial
+                              bounce-back per link), with the discrete adjoint of the drag;
+                              storage EUP_STORE_F64 or EUP_STORE_F32 */
 
 /* face codes of build.bcond.F90 (0 periodic, 4 inflow, 5 outflow) */
 #define EUP_FACE_PERIODIC 0
@@ -143,6 +146,30 @@ int eup_probe(eup_handle *h, int32_t i, int32_t j, int32_t k, double out[4]);
 /* Global means of diagno.F90:53-131: out = {<rho>, <u>, <v>, <w>, <u^2>}, solid sites counted as
  * rho = 1, u = 0.  FP64 reduction on the GPU (fixed order; not the reference's sequential order). */
 int eup_diagno(eup_handle *h, double out[5]);
+
+/* --- grey lattice and adjoint (EUP_MODE_GREY; no reference counterpart) --------------------- */
+/* The link between x and its source y = x - c_q carries sigma = 1 - (1 - s(x))(1 - s(y)) and the
+ * pulled population is (1 - sigma) a_q(y) + sigma a_qbar(x); s in {0, 1} is the physics mode's
+ * bounce-back.  Force F = sum over interior links of sigma c_q (a_q(y) - a_qbar(x)).             */
+
+/* Solid fraction s(1:l,1:m,1:n), i fastest, values in [0, 1] (host FP64 array). */
+int eup_set_grey(eup_handle *h, const double *s);
+
+/* As eup_set_grey from device memory of the handle's device; dtype EUP_STORE_F64 or _F32. */
+int eup_set_grey_device(eup_handle *h, const void *s, int32_t dtype);
+
+/* Force on the grey body from the current state. */
+int eup_grey_force(eup_handle *h, double F[3]);
+
+/* J = sum_{t=t0..t1} F_x(a^t) / ((t1 - t0 + 1) * norm), a^0 = the current state (left unchanged),
+ * and, if grad is not NULL, dJ/ds(1:l,1:m,1:n) by the discrete adjoint; forward states are
+ * checkpointed every `every` steps and recomputed per segment.  Host FP64 gradient. */
+int eup_grey_objective(eup_handle *h, int32_t t0, int32_t t1, int32_t every, double norm, double *J,
+                       double *grad);
+
+/* As eup_grey_objective with the gradient written to device memory (dtype EUP_STORE_F64/_F32). */
+int eup_grey_objective_device(eup_handle *h, int32_t t0, int32_t t1, int32_t every, double norm,
+                              double *J, void *grad, int32_t dtype);
 
 /* --- introspection ------------------------------------------------------------------------- */
 
