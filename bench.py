@@ -147,9 +147,16 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain MIB-MiB device copy (bandwidth / counter calibration)")
+    ap.add_argument("--membw", type=int, default=0, metavar="MIB",
+                    help="only run the library's copy / read / write HBM streams of MIB MiB")
     args = ap.parse_args(argv)
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.membw:
+        from euplectella_hpc_amd.lib import membw
+        r = membw(0, args.membw)
+        print(json.dumps({"membw_mib": args.membw, **{k: round(v, 1) for k, v in r.items()}}))
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not supported "
               "(the x-slab halo exchange is a design, not an implementation)", file=sys.stderr)

@@ -13,7 +13,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["eup_api.hip", "eup_parity.hip", "eup_physics.hip", "eup_diag.hip", "eup_grey.hip"]
+SOURCES = ["eup_api.hip", "eup_parity.hip", "eup_physics.hip", "eup_diag.hip", "eup_grey.hip",
+           "eup_membw.hip"]
 OUT = os.path.join(HERE, "libeup.so")
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -51,18 +52,22 @@ def build_fortran(verbose=False):
     return FORTRAN_EXE
 
 
-def build(verbose=False):
+def build(verbose=False, out=None, defines=(), fortran=True):
+    """Compile libeup.so; ``out``/``defines`` build a variant elsewhere (kernel experiments)."""
+    target = out or OUT
     srcs = [os.path.join(HERE, "csrc", s) for s in SOURCES]
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include")] + srcs + ["-o", OUT + ".tmp"]
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-D" + d for d in defines] + \
+        ["-I" + os.path.join(ROOT, "include")] + srcs + ["-o", target + ".tmp"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libeup.so")
-    os.replace(OUT + ".tmp", OUT)
+    os.replace(target + ".tmp", target)
     if verbose:
-        print("built", OUT)
-    build_fortran(verbose)
-    return OUT
+        print("built", target)
+    if fortran and out is None:
+        build_fortran(verbose)
+    return target
 
 
 if __name__ == "__main__":

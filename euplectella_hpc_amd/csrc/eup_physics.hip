@@ -97,10 +97,26 @@ __device__ __forceinline__ void pull_all(QSeq<Qs...>, const XArgs<S> &a, const R
     (pull<S, XOPEN, MIXED, Qs>(a, r, i, s, lm, x), ...);
 }
 
+#ifndef EUP_NT_STORE
+#define EUP_NT_STORE 1
+#endif
+
+template <typename S> __device__ __forceinline__ void store_val(S *p, S v)
+{
+#if EUP_NT_STORE
+    if constexpr (std::is_same_v<S, bf16_t>)
+        __builtin_nontemporal_store(v.bits, reinterpret_cast<uint16_t *>(p));
+    else
+        __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 template <typename S, int Q>
 __device__ __forceinline__ void put(const XArgs<S> &a, uint32_t s, const typename Store<S>::compute *x)
 {
-    a.B[Q][s] = Store<S>::store(x[Q], Q);
+    store_val<S>(&a.B[Q][s], Store<S>::store(x[Q], Q));
 }
 
 template <typename S, int... Qs>

@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libeup.so")
+LIB_PATH = os.environ.get("EUP_LIBRARY") or os.path.join(HERE, "libeup.so")
 
 EUP_OK = 0
 ERRORS = {-1: "EUP_EINVAL", -2: "EUP_EHIP", -3: "EUP_ESTATE", -4: "EUP_ERANGE"}
@@ -29,7 +29,7 @@ EXPORTS = ("eup_create", "eup_destroy", "eup_last_error", "eup_set_mask", "eup_s
            "eup_get_pops", "eup_set_state", "eup_get_state", "eup_step", "eup_step_timed",
            "eup_sync", "eup_drag_cv", "eup_drag_mem", "eup_probe", "eup_diagno", "eup_get_info",
            "eup_stream", "eup_abi_version", "eup_set_grey", "eup_set_grey_device",
-           "eup_grey_force", "eup_grey_objective", "eup_grey_objective_device")
+           "eup_grey_force", "eup_grey_objective", "eup_grey_objective_device", "eup_membw")
 
 
 class EupConfig(ctypes.Structure):
@@ -92,6 +92,7 @@ def load():
     L.eup_stream.restype = vp
     L.eup_abi_version.argtypes = []
     L.eup_abi_version.restype = i32
+    L.eup_membw.argtypes = [i32, ctypes.c_int64, i32, dp]
     L.eup_set_grey.argtypes = [vp, dp]
     L.eup_set_grey_device.argtypes = [vp, vp, i32]
     L.eup_grey_force.argtypes = [vp, dp]
@@ -285,6 +286,16 @@ class Lattice:
 
     def stream(self):
         return self.L.eup_stream(self.h)
+
+
+def membw(device=0, mib=4096, reps=10):
+    """HBM GB/s of copy / read / write streams of ``mib`` MiB buffers (eup_membw)."""
+    L = load()
+    out = np.zeros(3)
+    rc = L.eup_membw(int(device), int(mib) << 20, int(reps), _dptr(out))
+    if rc != EUP_OK:
+        raise EupError(rc, L.eup_last_error(None).decode())
+    return {"copy": out[0], "read": out[1], "write": out[2]}
 
 
 def drag_boxes(lx, ly, lz, radius=100, deltas=(52, 57, 62, 67)):

@@ -179,6 +179,11 @@ int eup_get_info(eup_handle *h, eup_info *info);
  * work (timing events, PyTorch) with the library's. */
 void *eup_stream(eup_handle *h);
 
+/* HBM bandwidth probe on `device`: GB/s of a copy (read + write), a read-only and a write-only
+ * stream over `bytes` per buffer, `reps` repetitions each: out = {copy, read, write}.
+ * Measurement utility (no reference counterpart). */
+int eup_membw(int32_t device, int64_t bytes, int32_t reps, double out[3]);
+
 /* ABI version of the loaded library (EUP_ABI_VERSION at build time). */
 int32_t eup_abi_version(void);
 
