@@ -267,6 +267,18 @@ int eup_get_pops(eup_handle *h, int32_t q, double *a)
     return 0;
 }
 
+int eup_init_equilibrium(eup_handle *h, double ux, double uy, double uz, double noise, uint64_t seed)
+{
+    if (!h) return fail(nullptr, EUP_EINVAL, "eup_init_equilibrium: null handle");
+    if (!(ux * ux + uy * uy + uz * uz < 0.09) || !(noise >= 0.0 && noise < 0.1))
+        return fail(h, EUP_EINVAL, "eup_init_equilibrium: |u| must stay below 0.3 and noise in [0, 0.1)");
+    if (hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice")) return EUP_EHIP;
+    int rc = init_equilibrium(h, ux, uy, uz, noise, seed);
+    if (rc) return rc;
+    h->pops_set = 0x7FFFFu;
+    return 0;
+}
+
 int eup_set_state(eup_handle *h, const double *a)
 {
     if (!h || !a) return fail(h, EUP_EINVAL, "eup_set_state: null argument");

@@ -78,6 +78,7 @@ int physics_refresh_outlet(Handle *h);
 int physics_set_pop(Handle *h, int q); // staging (ghosted f64) -> both buffers
 int physics_get_pop(Handle *h, int q); // current buffer -> staging
 int parity_set_pop(Handle *h, int q);
+int init_equilibrium(Handle *h, double ux, double uy, double uz, double noise, uint64_t seed);
 int parity_get_pop(Handle *h, int q);
 
 // grey mode (eup_grey.hip)

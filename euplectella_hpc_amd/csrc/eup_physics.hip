@@ -163,16 +163,54 @@ __device__ __forceinline__ void update_site(const XArgs<S> &a, int i, int j, int
     }
 }
 
+#ifndef EUP_PAIR
+#define EUP_PAIR 0
+#endif
+// tiles per wavefront: 1, or 2 adjacent tiles of one row (two independent loads per population
+// per lane: twice the bytes in flight per wave)
+constexpr int kTilesPerWave = EUP_PAIR ? 2 : 1;
+
 template <typename S, bool XOPEN>
-__global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
+__device__ __forceinline__ void update_pair(const XArgs<S> &a, int i, int j, int k)
 {
-    // XCD-aware block order: hardware block b runs on XCD b % 8; give each XCD a contiguous
-    // range of tiles so that the lines shared by neighbouring tiles stay in one L2.
-    const uint32_t b = blockIdx.x;
-    const uint32_t lb = (b & 7u) * a.chunk + (b >> 3);
-    const uint32_t t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kTile) + (threadIdx.x >> 6));
-    if (t >= a.tiles) return;
-    const int32_t code = __builtin_amdgcn_readfirstlane(a.tile_code[t]);
+    using R = typename Store<S>::compute;
+    const int m = a.m, n = a.n, l = a.l;
+    Rows r;
+    r.jj[0] = (j == 0) ? m - 1 : j - 1;
+    r.jj[1] = j;
+    r.jj[2] = (j == m - 1) ? 0 : j + 1;
+    r.kk[0] = (k == 0) ? n - 1 : k - 1;
+    r.kk[1] = k;
+    r.kk[2] = (k == n - 1) ? 0 : k + 1;
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+            r.off[dy][dz] = (uint32_t)l * (uint32_t)(r.jj[dy] + m * r.kk[dz]);
+    const uint32_t s0 = r.off[1][1] + (uint32_t)i, s1 = s0 + kTile;
+    R x0[19], x1[19];
+    pull_all<S, XOPEN, false>(Q18{}, a, r, i, s0, 0u, x0);
+    pull_all<S, XOPEN, false>(Q18{}, a, r, i + kTile, s1, 0u, x1);
+    x0[18] = Store<S>::load(a.A[18][s0], 18);
+    x1[18] = Store<S>::load(a.A[18][s1], 18);
+    collide<R>(x0, x0, a.omega, a.fgrad);
+    collide<R>(x1, x1, a.omega, a.fgrad);
+    put_all<S>(Q19{}, a, s0, x0);
+    put_all<S>(Q19{}, a, s1, x1);
+    if constexpr (XOPEN) {
+        if (i + kTile == l - 1) {
+            R o[5];
+            outlet_eq<R>(x1, o);
+            const size_t mn = (size_t)m * n;
+#pragma unroll
+            for (int u = 0; u < 5; ++u) a.out_next[u * mn + j + (size_t)m * k] = o[u];
+        }
+    }
+}
+
+template <typename S, bool XOPEN>
+__device__ __forceinline__ void do_tile(const XArgs<S> &a, uint32_t t, int32_t code)
+{
     if (code == kTileSolid) return;
     const uint32_t row = t / (uint32_t)a.tpr;
     const int i = (int)(t - row * (uint32_t)a.tpr) * kTile + (int)(threadIdx.x & 63);
@@ -184,6 +222,35 @@ __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
     } else {
         const uint32_t lm = a.link[(size_t)code * kTile + (threadIdx.x & 63)];
         update_site<S, XOPEN, true>(a, i, j, k, lm);
+    }
+}
+
+template <typename S, bool XOPEN>
+__global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
+{
+    // XCD-aware block order: hardware block b runs on XCD b % 8; give each XCD a contiguous
+    // range of tiles so that the lines shared by neighbouring tiles stay in one L2.
+    const uint32_t b = blockIdx.x;
+    const uint32_t lb = (b & 7u) * a.chunk + (b >> 3);
+    const uint32_t w = __builtin_amdgcn_readfirstlane(lb * (kBlock / kTile) + (threadIdx.x >> 6));
+    const uint32_t t0 = w * kTilesPerWave;
+    if (t0 >= a.tiles) return;
+    const int32_t c0 = __builtin_amdgcn_readfirstlane(a.tile_code[t0]);
+    if constexpr (kTilesPerWave == 1) {
+        do_tile<S, XOPEN>(a, t0, c0);
+    } else {
+        const uint32_t t1 = t0 + 1;
+        const int32_t c1 = t1 < a.tiles ? __builtin_amdgcn_readfirstlane(a.tile_code[t1]) : kTileSolid;
+        const uint32_t row = t0 / (uint32_t)a.tpr;
+        const uint32_t seg = t0 - row * (uint32_t)a.tpr;
+        if (c0 == kTileFluid && c1 == kTileFluid && seg + 1 < (uint32_t)a.tpr &&
+            (int)(seg + 2) * kTile <= a.l) {
+            const int i = (int)seg * kTile + (int)(threadIdx.x & 63);
+            update_pair<S, XOPEN>(a, i, (int)(row % (uint32_t)a.m), (int)(row / (uint32_t)a.m));
+        } else {
+            do_tile<S, XOPEN>(a, t0, c0);
+            if (t1 < a.tiles) do_tile<S, XOPEN>(a, t1, c1);
+        }
     }
 }
 
@@ -278,6 +345,37 @@ __global__ void k_to_staging(const S *__restrict__ src, double *__restrict__ stg
         (double)Store<S>::load(src[t], q);
 }
 
+// counter-based uniform in [-1, 1) (splitmix64 finaliser)
+__device__ __forceinline__ double hash_uniform(uint64_t seed, uint64_t i)
+{
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// equilibrium start; ghosted = 1: parity layout (l+2)(m+2)(n+2), else dense l*m*n
+template <typename S>
+__global__ void k_init_eq(S *base, int64_t stride, int64_t nsites, double ux, double uy0, double uz,
+                          double noise, uint64_t seed)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsites) return;
+    const double uy = uy0 + noise * hash_uniform(seed, (uint64_t)t);
+    const double usq = ux * ux + uy * uy + uz * uz;
+    const int cx[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+    const int cy[19] = {-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1, 0};
+    const int cz[19] = {0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1, 0};
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        const double cu = cx[q] * ux + cy[q] * uy + cz[q] * uz;
+        const double f = weight<double>(q) * (1.0 + (3.0 * cu + 1.5 * (3.0 * cu * cu - usq)));
+        using R = typename Store<S>::compute;
+        base[q * stride + t] = Store<S>::store((R)f, q);
+    }
+}
+
 template <typename S>
 __global__ void k_outlet_refresh(XArgs<S> a, typename Store<S>::compute *o0,
                                  typename Store<S>::compute *o1)
@@ -322,7 +420,8 @@ template <typename S> XArgs<S> make_args(Handle *h)
     a.n = h->g.n;
     a.tpr = h->g.tiles_per_row;
     a.tiles = (uint32_t)h->g.tiles;
-    const uint32_t nb = (uint32_t)((h->g.tiles + 3) / 4);
+    const uint32_t per_block = (kBlock / kTile) * kTilesPerWave;
+    const uint32_t nb = (uint32_t)((h->g.tiles + per_block - 1) / per_block);
     a.chunk = (nb + 7) / 8;
     return a;
 }
@@ -408,6 +507,24 @@ int physics_set_pop(Handle *h, int q)
 int physics_get_pop(Handle *h, int q)
 {
     return dispatch(h, [&](auto *p) { return get_t<std::remove_pointer_t<decltype(p)>>(h, q); });
+}
+
+int init_equilibrium(Handle *h, double ux, double uy, double uz, double noise, uint64_t seed)
+{
+    const bool parity = h->cfg.mode == EUP_MODE_PARITY;
+    const int64_t ns = parity ? h->g.ghosted : h->g.sites;
+    const unsigned grid = (unsigned)((ns + 255) / 256);
+    for (int b = 0; b < 2; ++b) {
+        int rc = dispatch(h, [&](auto *p) {
+            using S = std::remove_pointer_t<decltype(p)>;
+            hipLaunchKernelGGL((k_init_eq<S>), dim3(grid), dim3(256), 0, h->stream, static_cast<S *>(h->buf[b]),
+                               h->stride, ns, ux, uy, uz, noise, seed);
+            return hip_check(h, hipGetLastError(), "init equilibrium") ? EUP_EHIP : 0;
+        });
+        if (rc) return rc;
+    }
+    h->outlet_dirty = true;
+    return hip_check(h, hipStreamSynchronize(h->stream), "sync") ? EUP_EHIP : 0;
 }
 
 int physics_prepare_mask(Handle *h)

@@ -112,6 +112,13 @@ int eup_set_pops(eup_handle *h, int32_t q, const double *a);
  * (save_raw.F90:54-74).  Physics mode writes 0 into the ghost layer. */
 int eup_get_pops(eup_handle *h, int32_t q, double *a);
 
+/* Every site (ghosts included in parity mode) set to the equilibrium of rho = 1 and
+ * u = (ux, uy + noise * r, uz), r uniform in [-1, 1) from a counter-based hash of (seed, site).
+ * With ux = uy = uz = noise = 0 this is the reference's rest start (init.F90:128-236, opt 0);
+ * otherwise it is the device-side counterpart of a restore file holding an inlet-velocity start.
+ * Equilibrium form w*(1 + (3 c.u + 1.5 (3 (c.u)^2 - u^2))). */
+int eup_init_equilibrium(eup_handle *h, double ux, double uy, double uz, double noise, uint64_t seed);
+
 /* All 19 populations at once: a[(q-1)*(l+2)(m+2)(n+2) + offset]. */
 int eup_set_state(eup_handle *h, const double *a);
 int eup_get_state(eup_handle *h, double *a);

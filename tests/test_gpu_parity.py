@@ -204,3 +204,38 @@ def test_parity_diagno_and_mem_match_oracle(golden):
     assert np.allclose(d, orc.diagno(got, mask), rtol=1e-13, atol=1e-16)
     Fr = orc.mem_force(got, mask)
     assert np.abs(F - Fr).max() <= 1e-12 * np.abs(Fr).max(), (F, Fr)
+
+
+def hash_uniform(seed, n):
+    """numpy restatement of the library's counter-based uniform in [-1, 1) (splitmix64)."""
+    i = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + np.uint64(0x9E3779B97F4A7C15) * (i + np.uint64(1))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z ^= z >> np.uint64(31)
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+
+
+@pytest.mark.parametrize("mode", ["parity", "physics"])
+def test_init_equilibrium_matches_the_closed_form(mode):
+    from euplectella_hpc_amd import d3q19
+    l, m, n = 20, 12, 10
+    announce(f"init_equilibrium {mode} {l}x{m}x{n}")
+    ux, noise, seed = 0.06, 1e-4, 77
+    with lattice((l, m, n), svisc=0.05, u_inflow=ux, mode=mode, storage="f64") as lat:
+        lat.set_mask(np.zeros((n, m, l), np.uint8))
+        lat.init_equilibrium(ux, 0.0, 0.0, noise, seed)
+        got = lat.get_state()
+    if mode == "parity":
+        uy = (noise * hash_uniform(seed, (n + 2) * (m + 2) * (l + 2))).reshape(n + 2, m + 2, l + 2)
+        region = (slice(None),) * 4
+    else:
+        uy = np.zeros((n + 2, m + 2, l + 2))
+        uy[1:-1, 1:-1, 1:-1] = (noise * hash_uniform(seed, n * m * l)).reshape(n, m, l)
+        region = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+    usq = ux * ux + uy * uy
+    for q in range(19):
+        cu = d3q19.C[q, 0] * ux + d3q19.C[q, 1] * uy
+        e = d3q19.W[q] * (1.0 + (3.0 * cu + 1.5 * (3.0 * cu * cu - usq)))
+        assert np.abs(got[q][region[1:]] - e[region[1:]]).max() < 1e-17, q
