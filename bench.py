@@ -28,19 +28,31 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
+    # cylinder R = 8 at (32, 32), tau 0.6, u_in 0.05
+    "cylinder_128x64x64": dict(kind="cylinder", l=128, m=64, n=64, radius=8, centre=(32, 32),
+                               seed=2026, tau=0.6, u_in=0.05, diameter=16.0),
+    # cylinder R = 12 with 8 helical 3 x 3 ridges at 45 degrees, tau 0.52, u_in 0.08
+    "ridged_256x128x128": dict(kind="ridged", l=256, m=128, n=128, radius=12, centre=(64, 64),
+                               seed=2026, tau=0.52, u_in=0.08, ridge_width=3, diameter=30.0),
     # reticular sponge: hollow tube R = 48, wall 3, strut cell 16, tau 0.505, u_in 0.06
     "sponge_512x256x256": dict(l=512, m=256, n=256, radius=48, centre=(128, 128), cell=16,
-                               thickness=3.0, seed=2026, tau=0.505, u_in=0.06, ridges=0),
+                               thickness=3.0, seed=2026, tau=0.505, u_in=0.06, ridges=0,
+                               diameter=99.0),
     # capacity run: the same generator scaled by 3 (R = 144, wall 9, cell 48) with 8 helical
     # 9 x 9 ridges on the outer surface, tau 0.502, u_in 0.05; 906 M sites
     "sponge_1536x768x768": dict(l=1536, m=768, n=768, radius=144, centre=(384, 384), cell=48,
                                 thickness=9.0, seed=2026, tau=0.502, u_in=0.05, ridges=8,
-                                ridge_width=9),
+                                ridge_width=9, diameter=315.0),
 }
 
 
 def workload_mask(wl, nz=None):
     from euplectella_hpc_amd import geometry
+    if wl.get("kind") == "cylinder":
+        return geometry.cylinder(wl["l"], wl["m"], nz or wl["n"], wl["radius"], wl["centre"])
+    if wl.get("kind") == "ridged":
+        return geometry.ridged_cylinder(wl["l"], wl["m"], nz or wl["n"], wl["radius"], wl["centre"],
+                                        n_ridges=8, width=wl["ridge_width"])
     return geometry.sponge(wl["l"], wl["m"], nz or wl["n"], wl["radius"], wl["centre"],
                            cell=wl["cell"], thickness=wl["thickness"], seed=wl["seed"],
                            ridges=wl.get("ridges", 0), ridge_width=wl.get("ridge_width", 3))
@@ -148,6 +160,52 @@ def copy_probe(mib, reps=10):
     return 0
 
 
+def accuracy(wl, storages, steps, every, window=0.5):
+    """Drag coefficient and probe of reduced storage against an FP64 run of the same workload.
+
+    C_D = F_x / (0.5 rho0 u_in^2 D H) with F the momentum-exchange force (eup_drag_mem) every
+    ``every`` steps, D the body's frontal diameter and H = lz; reported: time-mean C_D over the
+    last ``window`` of the run and |dC_D|/C_D against FP64, and the max probe velocity error.
+    """
+    mask = workload_mask(wl)
+    probe = (wl["l"] * 8 // 20, wl["m"] * 9 // 16, wl["n"] // 2 - 5)
+    norm = 0.5 * wl["u_in"] ** 2 * wl["diameter"] * wl["n"]
+    hist = {}
+    for st in ["f64"] + [x for x in storages if x != "f64"]:
+        lat = build_lattice(wl, st, mask)
+        cd, pr = [], []
+        t0 = time.perf_counter()
+        try:
+            done = 0
+            while done < steps:
+                k = min(every, steps - done)
+                lat.step(k)
+                done += k
+                cd.append(lat.drag_mem()[0] / norm)
+                pr.append(lat.probe(*probe)[:3])
+        finally:
+            lat.close()
+        hist[st] = (np.array(cd), np.array(pr))
+        print(f"[accuracy] {st}: {steps} steps in {time.perf_counter() - t0:.1f} s, "
+              f"C_D(end) = {cd[-1]:.6f}", file=sys.stderr, flush=True)
+    cut = int(len(hist["f64"][0]) * (1 - window))
+    ref_cd = hist["f64"][0][cut:].mean()
+    out = {"workload": wl_name(wl), "steps": steps, "every": every, "cd_f64": ref_cd,
+           "finite_f64": bool(np.all(np.isfinite(hist["f64"][0])))}
+    for st, (cd, pr) in hist.items():
+        if st == "f64":
+            continue
+        m = cd[cut:].mean()
+        out[st] = {"cd": m, "rel_dcd": abs(m - ref_cd) / abs(ref_cd),
+                   "max_rel_dcd_inst": float(np.max(np.abs(cd - hist["f64"][0]) / abs(ref_cd))),
+                   "probe_max_abs_du": float(np.max(np.abs(pr - hist["f64"][1])))}
+    return out
+
+
+def wl_name(wl):
+    return next(k for k, v in WORKLOADS.items() if v is wl)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,7 +219,14 @@ def main(argv=None):
                     help="only run a plain MIB-MiB device copy (bandwidth / counter calibration)")
     ap.add_argument("--membw", type=int, default=0, metavar="MIB",
                     help="only run the library's copy / read / write HBM streams of MIB MiB")
+    ap.add_argument("--accuracy", type=int, default=0, metavar="STEPS",
+                    help="only run the C_D / probe accuracy study of f32, f16, bf16 vs f64")
+    ap.add_argument("--every", type=int, default=100, help="drag sampling interval (--accuracy)")
     args = ap.parse_args(argv)
+    if args.accuracy:
+        r = accuracy(WORKLOADS[args.workload], ["f32", "f16", "bf16"], args.accuracy, args.every)
+        print(json.dumps(r))
+        return 0
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.membw:
