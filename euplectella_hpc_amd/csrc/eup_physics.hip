@@ -163,15 +163,15 @@ __device__ __forceinline__ void update_site(const XArgs<S> &a, int i, int j, int
     }
 }
 
-#ifndef EUP_PAIR
-#define EUP_PAIR 0
+#ifndef EUP_TPW
+#define EUP_TPW 1
 #endif
-// tiles per wavefront: 1, or 2 adjacent tiles of one row (two independent loads per population
-// per lane: twice the bytes in flight per wave)
-constexpr int kTilesPerWave = EUP_PAIR ? 2 : 1;
+// tiles per wavefront: NT adjacent pure-fluid tiles of one row are updated together (NT
+// independent loads per population per lane in flight); other tiles one at a time
+constexpr int kTilesPerWave = EUP_TPW;
 
-template <typename S, bool XOPEN>
-__device__ __forceinline__ void update_pair(const XArgs<S> &a, int i, int j, int k)
+template <typename S, bool XOPEN, int NT>
+__device__ __forceinline__ void update_multi(const XArgs<S> &a, int i, int j, int k)
 {
     using R = typename Store<S>::compute;
     const int m = a.m, n = a.n, l = a.l;
@@ -187,20 +187,22 @@ __device__ __forceinline__ void update_pair(const XArgs<S> &a, int i, int j, int
 #pragma unroll
         for (int dz = 0; dz < 3; ++dz)
             r.off[dy][dz] = (uint32_t)l * (uint32_t)(r.jj[dy] + m * r.kk[dz]);
-    const uint32_t s0 = r.off[1][1] + (uint32_t)i, s1 = s0 + kTile;
-    R x0[19], x1[19];
-    pull_all<S, XOPEN, false>(Q18{}, a, r, i, s0, 0u, x0);
-    pull_all<S, XOPEN, false>(Q18{}, a, r, i + kTile, s1, 0u, x1);
-    x0[18] = Store<S>::load(a.A[18][s0], 18);
-    x1[18] = Store<S>::load(a.A[18][s1], 18);
-    collide<R>(x0, x0, a.omega, a.fgrad);
-    collide<R>(x1, x1, a.omega, a.fgrad);
-    put_all<S>(Q19{}, a, s0, x0);
-    put_all<S>(Q19{}, a, s1, x1);
+    const uint32_t s0 = r.off[1][1] + (uint32_t)i;
+    R x[NT][19];
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+        pull_all<S, XOPEN, false>(Q18{}, a, r, i + u * kTile, s0 + u * kTile, 0u, x[u]);
+        x[u][18] = Store<S>::load(a.A[18][s0 + u * kTile], 18);
+    }
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+        collide<R>(x[u], x[u], a.omega, a.fgrad);
+        put_all<S>(Q19{}, a, s0 + u * kTile, x[u]);
+    }
     if constexpr (XOPEN) {
-        if (i + kTile == l - 1) {
+        if (i + (NT - 1) * kTile == l - 1) {
             R o[5];
-            outlet_eq<R>(x1, o);
+            outlet_eq<R>(x[NT - 1], o);
             const size_t mn = (size_t)m * n;
 #pragma unroll
             for (int u = 0; u < 5; ++u) a.out_next[u * mn + j + (size_t)m * k] = o[u];
@@ -239,17 +241,23 @@ __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
     if constexpr (kTilesPerWave == 1) {
         do_tile<S, XOPEN>(a, t0, c0);
     } else {
-        const uint32_t t1 = t0 + 1;
-        const int32_t c1 = t1 < a.tiles ? __builtin_amdgcn_readfirstlane(a.tile_code[t1]) : kTileSolid;
         const uint32_t row = t0 / (uint32_t)a.tpr;
         const uint32_t seg = t0 - row * (uint32_t)a.tpr;
-        if (c0 == kTileFluid && c1 == kTileFluid && seg + 1 < (uint32_t)a.tpr &&
-            (int)(seg + 2) * kTile <= a.l) {
+        bool all_fluid = c0 == kTileFluid && seg + kTilesPerWave <= (uint32_t)a.tpr &&
+                         (int)(seg + kTilesPerWave) * kTile <= a.l;
+#pragma unroll
+        for (int u = 1; u < kTilesPerWave; ++u)
+            all_fluid = all_fluid && t0 + u < a.tiles && a.tile_code[t0 + u] == kTileFluid;
+        if (__builtin_amdgcn_readfirstlane((int)all_fluid)) {
             const int i = (int)seg * kTile + (int)(threadIdx.x & 63);
-            update_pair<S, XOPEN>(a, i, (int)(row % (uint32_t)a.m), (int)(row / (uint32_t)a.m));
+            update_multi<S, XOPEN, kTilesPerWave>(a, i, (int)(row % (uint32_t)a.m),
+                                                  (int)(row / (uint32_t)a.m));
         } else {
             do_tile<S, XOPEN>(a, t0, c0);
-            if (t1 < a.tiles) do_tile<S, XOPEN>(a, t1, c1);
+#pragma unroll
+            for (int u = 1; u < kTilesPerWave; ++u)
+                if (t0 + u < a.tiles)
+                    do_tile<S, XOPEN>(a, t0 + u, __builtin_amdgcn_readfirstlane(a.tile_code[t0 + u]));
         }
     }
 }
