@@ -1,0 +1,26 @@
+"""bench.py command-line contract that does not need a GPU."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    res = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 2
+    assert "one GPU only" in res.stderr
+    assert res.stdout.strip() == ""
+
+
+def test_workloads_define_the_configured_lattices():
+    sys.path.insert(0, ROOT)
+    import bench
+
+    wl = bench.WORKLOADS["sponge_512x256x256"]
+    assert (wl["l"], wl["m"], wl["n"], wl["tau"], wl["u_in"]) == (512, 256, 256, 0.505, 0.06)
+    c5 = bench.WORKLOADS["sponge_1536x768x768"]
+    assert (c5["l"], c5["m"], c5["n"], c5["radius"], c5["cell"]) == (1536, 768, 768, 144, 48)
+    c1 = bench.WORKLOADS["cylinder_128x64x64"]
+    assert (c1["radius"], c1["centre"], c1["tau"], c1["u_in"]) == (8, (32, 32), 0.6, 0.05)
