@@ -163,12 +163,18 @@ __device__ __forceinline__ void update_site(const XArgs<S> &a, int i, int j, int
     }
 }
 
-#ifndef EUP_TPW
-#define EUP_TPW 1
+// Tiles per wavefront: NT adjacent pure-fluid tiles of one row are updated together (NT
+// independent loads per population per lane in flight); other tiles one at a time.  Two tiles
+// per wave pay off for the 2-byte storages, whose single-tile loads move only 128 B per
+// instruction; 4- and 8-byte storages run one tile per wave.
+template <typename S> struct TilesPerWave {
+    static constexpr int value = sizeof(S) == 2 ? 2 : 1;
+};
+#ifdef EUP_TPW
+#define EUP_TPW_OF(S) EUP_TPW
+#else
+#define EUP_TPW_OF(S) TilesPerWave<S>::value
 #endif
-// tiles per wavefront: NT adjacent pure-fluid tiles of one row are updated together (NT
-// independent loads per population per lane in flight); other tiles one at a time
-constexpr int kTilesPerWave = EUP_TPW;
 
 template <typename S, bool XOPEN, int NT>
 __device__ __forceinline__ void update_multi(const XArgs<S> &a, int i, int j, int k)
@@ -230,6 +236,7 @@ __device__ __forceinline__ void do_tile(const XArgs<S> &a, uint32_t t, int32_t c
 template <typename S, bool XOPEN>
 __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
 {
+    constexpr int kTilesPerWave = EUP_TPW_OF(S);
     // XCD-aware block order: hardware block b runs on XCD b % 8; give each XCD a contiguous
     // range of tiles so that the lines shared by neighbouring tiles stay in one L2.
     const uint32_t b = blockIdx.x;
@@ -428,7 +435,7 @@ template <typename S> XArgs<S> make_args(Handle *h)
     a.n = h->g.n;
     a.tpr = h->g.tiles_per_row;
     a.tiles = (uint32_t)h->g.tiles;
-    const uint32_t per_block = (kBlock / kTile) * kTilesPerWave;
+    const uint32_t per_block = (kBlock / kTile) * EUP_TPW_OF(S);
     const uint32_t nb = (uint32_t)((h->g.tiles + per_block - 1) / per_block);
     a.chunk = (nb + 7) / 8;
     return a;

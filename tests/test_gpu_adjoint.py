@@ -125,12 +125,14 @@ def test_c4_fp32_adjoint_matches_fp64_adjoint_and_finite_differences():
         # central differences of the FP64 objective on cells inside the grey transition zone
         g = grads["f64"]
         rng = np.random.default_rng(0)
-        # cells in the grey transition zone (both central-difference points inside [0, 1]) with a
-        # gradient of at least 5 % of the largest one
+        # cells whose central-difference points stay inside [0, 1], with a gradient of at least
+        # 1 % of the largest one (the largest gradients sit at the fluid edge of the blur, s << 1)
         sf = s.ravel()
-        cand = np.flatnonzero((np.abs(g.ravel()) > 0.05 * np.abs(g).max()) & (sf > 0.02) & (sf < 0.98))
-        cells = rng.choice(cand, min(12, cand.size), replace=False)
-        eps = 1e-4
+        eps = 1e-5
+        cand = np.flatnonzero((np.abs(g.ravel()) > 0.01 * np.abs(g).max()) & (sf > 4 * eps) &
+                              (sf < 1 - 4 * eps))
+        assert cand.size >= 12
+        cells = rng.choice(cand, 12, replace=False)
         worst = 0.0
         for cidx in cells:
             Js = []
