@@ -59,15 +59,21 @@ def workload_mask(wl, nz=None):
 
 
 def storage_bytes(storage):
-    return {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}[storage]
+    return {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16s": 2, "bf16s": 2}[storage]
+
+
+def parse_storage(spec):
+    """'f16s' / 'bf16s' = FP16 / BF16 deviation from the free-stream equilibrium."""
+    return (spec[:-1], True) if spec in ("f16s", "bf16s") else (spec, False)
 
 
 def build_lattice(wl, storage, mask):
     from euplectella_hpc_amd.lib import Lattice, svisc_from_tau
 
     l, m, n = wl["l"], wl["m"], wl["n"]
+    st, shifted = parse_storage(storage)
     lat = Lattice(l, m, n, svisc=svisc_from_tau(wl["tau"]), u_inflow=wl["u_in"], mode="physics",
-                  storage=storage)
+                  storage=st, shifted=shifted)
     lat.set_mask(mask)
     # u = (u_in, +-1e-4 seeded noise, 0) at every site, generated on the device
     lat.init_equilibrium(wl["u_in"], 0.0, 0.0, 1e-4, wl["seed"])
@@ -213,7 +219,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="sponge_512x256x256", choices=sorted(WORKLOADS))
-    ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16", "bf16"])
+    ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16", "bf16", "f16s", "bf16s"])
     ap.add_argument("--secondary", default="f16", help="second storage to report ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -225,7 +231,8 @@ def main(argv=None):
     ap.add_argument("--every", type=int, default=100, help="drag sampling interval (--accuracy)")
     args = ap.parse_args(argv)
     if args.accuracy:
-        r = accuracy(WORKLOADS[args.workload], ["f32", "f16", "bf16"], args.accuracy, args.every)
+        r = accuracy(WORKLOADS[args.workload], ["f32", "f16", "f16s", "bf16", "bf16s"], args.accuracy,
+                     args.every)
         print(json.dumps(r))
         return 0
     if args.copy_probe:
@@ -266,7 +273,8 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f64": "f64", "f32": "f32", "f16": "f16-dev/f32", "bf16": "bf16-dev/f32"}[args.storage],
+        "dtype": {"f64": "f64", "f32": "f32", "f16": "f16-dev/f32", "bf16": "bf16-dev/f32",
+                  "f16s": "f16-dev(u_in)/f32", "bf16s": "bf16-dev(u_in)/f32"}[args.storage],
         "data": "synthetic (seeded sponge geometry, u = inlet + seeded noise)",
         "config": {"workload": args.workload, "lattice": [wl["l"], wl["m"], wl["n"]],
                    "tau": wl["tau"], "u_in": wl["u_in"], "storage": args.storage,

@@ -30,10 +30,11 @@ struct ParityAcc {
 template <typename S> struct PhysAcc {
     using value_type = S;
     const S *p[19];
+    DevRef ref;
     int l, m, n;
     __device__ __forceinline__ double operator()(int q, int i, int j, int k) const
     {
-        return (double)Store<S>::load(p[q][(int64_t)(i - 1) + (int64_t)l * ((int64_t)(j - 1) + (int64_t)m * (k - 1))], q);
+        return (double)Store<S>::load(p[q][(int64_t)(i - 1) + (int64_t)l * ((int64_t)(j - 1) + (int64_t)m * (k - 1))], ref.r[q]);
     }
 };
 
@@ -262,6 +263,7 @@ template <typename S> PhysAcc<S> phys_acc(Handle *h)
     PhysAcc<S> a;
     const S *base = static_cast<const S *>(h->buf[h->cur]);
     for (int q = 0; q < 19; ++q) a.p[q] = base + (int64_t)q * h->stride;
+    a.ref = dev_ref(h);
     a.l = h->g.l;
     a.m = h->g.m;
     a.n = h->g.n;

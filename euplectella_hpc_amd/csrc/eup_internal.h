@@ -94,6 +94,21 @@ int diag_drag_cv(Handle *h, const int32_t box[6], double *fx, double *fy);
 int diag_drag_mem(Handle *h, double F[3]);
 int diag_diagno(Handle *h, double out[5]);
 
+// deviation reference of the handle: w_q, or f_eq(rho = 1, u = (u_inflow, 0, 0)) with
+// EUP_FLAG_SHIFTED_DEVIATION
+inline DevRef dev_ref(const Handle *h)
+{
+    DevRef d;
+    const bool shifted = (h->cfg.flags & EUP_FLAG_SHIFTED_DEVIATION) != 0;
+    const double u = shifted ? h->cfg.u_inflow : 0.0;
+    const int cx[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+    for (int q = 0; q < 19; ++q) {
+        const double cu = cx[q] * u;
+        d.r[q] = (float)(weight<double>(q) * (1.0 + (3.0 * cu + 1.5 * (3.0 * cu * cu - u * u))));
+    }
+    return d;
+}
+
 inline int64_t ghosted_index(const Geometry &g, int i, int j, int k)
 {
     return (int64_t)i + (int64_t)(g.l + 2) * ((int64_t)j + (int64_t)(g.m + 2) * (int64_t)k);

@@ -187,29 +187,31 @@ struct bf16_t {
 
 template <typename S> struct Store;
 
+// `ref` is the reference value of the population for deviation storage (w_q, or the free-stream
+// equilibrium of that population); full-value storages ignore it.
 template <> struct Store<double> {
     using compute = double;
     static constexpr bool deviation = false;
-    static __device__ __forceinline__ double load(double v, int) { return v; }
-    static __device__ __forceinline__ double store(double f, int) { return f; }
+    static __device__ __forceinline__ double load(double v, float) { return v; }
+    static __device__ __forceinline__ double store(double f, float) { return f; }
 };
 
 template <> struct Store<float> {
     using compute = float;
     static constexpr bool deviation = false;
-    static __device__ __forceinline__ float load(float v, int) { return v; }
-    static __device__ __forceinline__ float store(float f, int) { return f; }
+    static __device__ __forceinline__ float load(float v, float) { return v; }
+    static __device__ __forceinline__ float store(float f, float) { return f; }
 };
 
-// deviation f - w_q in binary16; float -> half conversion rounds to nearest even
+// deviation f - ref in binary16; float -> half conversion rounds to nearest even
 template <> struct Store<_Float16> {
     using compute = float;
     static constexpr bool deviation = true;
-    static __device__ __forceinline__ float load(_Float16 v, int q) { return weight<float>(q) + (float)v; }
-    static __device__ __forceinline__ _Float16 store(float f, int q) { return (_Float16)(f - weight<float>(q)); }
+    static __device__ __forceinline__ float load(_Float16 v, float ref) { return ref + (float)v; }
+    static __device__ __forceinline__ _Float16 store(float f, float ref) { return (_Float16)(f - ref); }
 };
 
-// deviation f - w_q in bfloat16 with round-to-nearest-even
+// deviation f - ref in bfloat16 with round-to-nearest-even
 template <> struct Store<bf16_t> {
     using compute = float;
     static constexpr bool deviation = true;
@@ -222,8 +224,15 @@ template <> struct Store<bf16_t> {
         b.bits = (uint16_t)(r >> 16);
         return b;
     }
-    static __device__ __forceinline__ float load(bf16_t v, int q) { return weight<float>(q) + to_float(v); }
-    static __device__ __forceinline__ bf16_t store(float f, int q) { return from_float(f - weight<float>(q)); }
+    static __device__ __forceinline__ float load(bf16_t v, float ref) { return ref + to_float(v); }
+    static __device__ __forceinline__ bf16_t store(float f, float ref) { return from_float(f - ref); }
 };
 
+} // namespace eup
+
+namespace eup {
+// reference values of the 19 populations for deviation storage (kernel argument)
+struct DevRef {
+    float r[19];
+};
 } // namespace eup

@@ -34,6 +34,7 @@ template <typename S> struct XArgs {
     const R *__restrict__ out_cur; // 5 x (m*n)
     R *__restrict__ out_next;
     R inl[5];
+    DevRef ref;
     R omega, fgrad;
     int l, m, n, tpr;
     uint32_t tiles;
@@ -69,13 +70,15 @@ __device__ __forceinline__ void pull(const XArgs<S> &a, const Rows &r, int i, ui
     const uint32_t src = r.off[1 - cy][1 - cz] + (uint32_t)si;
     uint32_t addr_q = src;
     const S *base = a.A[Q];
+    float refq = a.ref.r[Q];
     if constexpr (MIXED) {
         if ((lm >> Q) & 1u) { // the source is solid: bounce back the site's own population
             base = a.A[op];
             addr_q = s;
+            refq = a.ref.r[op];
         }
     }
-    R v = Store<S>::load(base[addr_q], Q);
+    R v = Store<S>::load(base[addr_q], refq);
     if constexpr (XOPEN) {
         if constexpr (cx == 1) {
             if (i == 0) v = a.inl[Q];
@@ -116,7 +119,7 @@ template <typename S> __device__ __forceinline__ void store_val(S *p, S v)
 template <typename S, int Q>
 __device__ __forceinline__ void put(const XArgs<S> &a, uint32_t s, const typename Store<S>::compute *x)
 {
-    store_val<S>(&a.B[Q][s], Store<S>::store(x[Q], Q));
+    store_val<S>(&a.B[Q][s], Store<S>::store(x[Q], a.ref.r[Q]));
 }
 
 template <typename S, int... Qs>
@@ -149,7 +152,7 @@ __device__ __forceinline__ void update_site(const XArgs<S> &a, int i, int j, int
     const uint32_t s = r.off[1][1] + (uint32_t)i;
     R x[19];
     pull_all<S, XOPEN, MIXED>(Q18{}, a, r, i, s, lm, x);
-    x[18] = Store<S>::load(a.A[18][s], 18);
+    x[18] = Store<S>::load(a.A[18][s], a.ref.r[18]);
     collide<R>(x, x, a.omega, a.fgrad);
     put_all<S>(Q19{}, a, s, x);
     if constexpr (XOPEN) {
@@ -198,7 +201,7 @@ __device__ __forceinline__ void update_multi(const XArgs<S> &a, int i, int j, in
 #pragma unroll
     for (int u = 0; u < NT; ++u) {
         pull_all<S, XOPEN, false>(Q18{}, a, r, i + u * kTile, s0 + u * kTile, 0u, x[u]);
-        x[u][18] = Store<S>::load(a.A[18][s0 + u * kTile], 18);
+        x[u][18] = Store<S>::load(a.A[18][s0 + u * kTile], a.ref.r[18]);
     }
 #pragma unroll
     for (int u = 0; u < NT; ++u) {
@@ -331,7 +334,7 @@ __global__ void k_compact_links(const int32_t *__restrict__ code, const uint32_t
 // ---------------------------------------------------------------------------------------------
 template <typename S>
 __global__ void k_from_staging(const double *__restrict__ stg, S *__restrict__ d0, S *__restrict__ d1,
-                               int l, int m, int n, int q)
+                               int l, int m, int n, int q, float ref)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t N = (int64_t)l * m * n;
@@ -341,14 +344,14 @@ __global__ void k_from_staging(const double *__restrict__ stg, S *__restrict__ d
     const int j = (int)(r % m), k = (int)(r / m);
     const double v = stg[(int64_t)(i + 1) + (int64_t)(l + 2) * ((int64_t)(j + 1) + (int64_t)(m + 2) * (k + 1))];
     using R = typename Store<S>::compute;
-    const S sv = Store<S>::store((R)v, q);
+    const S sv = Store<S>::store((R)v, ref);
     d0[t] = sv;
     d1[t] = sv;
 }
 
 template <typename S>
 __global__ void k_to_staging(const S *__restrict__ src, double *__restrict__ stg, int l, int m, int n,
-                             int q)
+                             int q, float ref)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t N = (int64_t)l * m * n;
@@ -357,7 +360,7 @@ __global__ void k_to_staging(const S *__restrict__ src, double *__restrict__ stg
     const int64_t r = t / l;
     const int j = (int)(r % m), k = (int)(r / m);
     stg[(int64_t)(i + 1) + (int64_t)(l + 2) * ((int64_t)(j + 1) + (int64_t)(m + 2) * (k + 1))] =
-        (double)Store<S>::load(src[t], q);
+        (double)Store<S>::load(src[t], ref);
 }
 
 // counter-based uniform in [-1, 1) (splitmix64 finaliser)
@@ -373,7 +376,7 @@ __device__ __forceinline__ double hash_uniform(uint64_t seed, uint64_t i)
 // equilibrium start; ghosted = 1: parity layout (l+2)(m+2)(n+2), else dense l*m*n
 template <typename S>
 __global__ void k_init_eq(S *base, int64_t stride, int64_t nsites, double ux, double uy0, double uz,
-                          double noise, uint64_t seed)
+                          double noise, uint64_t seed, DevRef ref)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsites) return;
@@ -387,7 +390,7 @@ __global__ void k_init_eq(S *base, int64_t stride, int64_t nsites, double ux, do
         const double cu = cx[q] * ux + cy[q] * uy + cz[q] * uz;
         const double f = weight<double>(q) * (1.0 + (3.0 * cu + 1.5 * (3.0 * cu * cu - usq)));
         using R = typename Store<S>::compute;
-        base[q * stride + t] = Store<S>::store((R)f, q);
+        base[q * stride + t] = Store<S>::store((R)f, ref.r[q]);
     }
 }
 
@@ -402,7 +405,7 @@ __global__ void k_outlet_refresh(XArgs<S> a, typename Store<S>::compute *o0,
     const uint32_t s = (uint32_t)(a.l - 1) + (uint32_t)a.l * (uint32_t)t;
     R x[19], o[5];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) x[q] = Store<S>::load(a.A[q][s], q);
+    for (int q = 0; q < 19; ++q) x[q] = Store<S>::load(a.A[q][s], a.ref.r[q]);
     outlet_eq<R>(x, o);
 #pragma unroll
     for (int u = 0; u < 5; ++u) {
@@ -428,6 +431,7 @@ template <typename S> XArgs<S> make_args(Handle *h)
     R inl[5];
     inlet_eq<R>((R)h->cfg.u_inflow, inl);
     for (int u = 0; u < 5; ++u) a.inl[u] = inl[u];
+    a.ref = dev_ref(h);
     a.omega = (R)h->cfg.omega;
     a.fgrad = (R)h->cfg.fgrad;
     a.l = h->g.l;
@@ -471,7 +475,7 @@ template <typename S> int set_t(Handle *h, int q)
     S *d0 = static_cast<S *>(h->buf[0]) + (int64_t)(q - 1) * h->stride;
     S *d1 = static_cast<S *>(h->buf[1]) + (int64_t)(q - 1) * h->stride;
     hipLaunchKernelGGL((k_from_staging<S>), dim3((unsigned)((h->g.sites + 255) / 256)), dim3(256), 0,
-                       h->stream, h->staging, d0, d1, h->g.l, h->g.m, h->g.n, q - 1);
+                       h->stream, h->staging, d0, d1, h->g.l, h->g.m, h->g.n, q - 1, dev_ref(h).r[q - 1]);
     return hip_check(h, hipGetLastError(), "set pops") ? EUP_EHIP : 0;
 }
 
@@ -481,7 +485,7 @@ template <typename S> int get_t(Handle *h, int q)
     if (hip_check(h, hipMemsetAsync(h->staging, 0, sizeof(double) * h->g.ghosted, h->stream), "memset"))
         return EUP_EHIP;
     hipLaunchKernelGGL((k_to_staging<S>), dim3((unsigned)((h->g.sites + 255) / 256)), dim3(256), 0,
-                       h->stream, s, h->staging, h->g.l, h->g.m, h->g.n, q - 1);
+                       h->stream, s, h->staging, h->g.l, h->g.m, h->g.n, q - 1, dev_ref(h).r[q - 1]);
     return hip_check(h, hipGetLastError(), "get pops") ? EUP_EHIP : 0;
 }
 
@@ -533,7 +537,7 @@ int init_equilibrium(Handle *h, double ux, double uy, double uz, double noise, u
         int rc = dispatch(h, [&](auto *p) {
             using S = std::remove_pointer_t<decltype(p)>;
             hipLaunchKernelGGL((k_init_eq<S>), dim3(grid), dim3(256), 0, h->stream, static_cast<S *>(h->buf[b]),
-                               h->stride, ns, ux, uy, uz, noise, seed);
+                               h->stride, ns, ux, uy, uz, noise, seed, dev_ref(h));
             return hip_check(h, hipGetLastError(), "init equilibrium") ? EUP_EHIP : 0;
         });
         if (rc) return rc;

@@ -23,6 +23,7 @@ MODE_PARITY, MODE_PHYSICS, MODE_GREY = 0, 1, 2
 MODES = {"parity": MODE_PARITY, "physics": MODE_PHYSICS, "grey": MODE_GREY}
 FACE_PERIODIC, FACE_INFLOW, FACE_OUTFLOW = 0, 4, 5
 FLAG_NO_TILE_SKIP = 1
+FLAG_SHIFTED_DEVIATION = 2
 
 #: every function the header declares (checked by the CPU tests)
 EXPORTS = ("eup_create", "eup_destroy", "eup_last_error", "eup_set_mask", "eup_set_pops",
@@ -135,7 +136,7 @@ class Lattice:
     """
 
     def __init__(self, l, m, n, *, svisc, u_inflow, mode="physics", storage="f32", x_open=True,
-                 fgrad=0.0, device=0, tile_skip=True):
+                 fgrad=0.0, device=0, tile_skip=True, shifted=False):
         self.L = load()
         self.shape = (l, m, n)
         self.svisc = float(svisc)
@@ -149,7 +150,7 @@ class Lattice:
         cfg.rear = FACE_INFLOW if x_open else FACE_PERIODIC
         cfg.up = cfg.down = cfg.left = cfg.right = FACE_PERIODIC
         cfg.device = device
-        cfg.flags = 0 if tile_skip else FLAG_NO_TILE_SKIP
+        cfg.flags = (0 if tile_skip else FLAG_NO_TILE_SKIP) | (FLAG_SHIFTED_DEVIATION if shifted else 0)
         cfg.omega = self.omega
         cfg.fgrad = float(fgrad)
         cfg.u_inflow = self.u_inflow
