@@ -30,20 +30,28 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     # cylinder R = 8 at (32, 32), tau 0.6, u_in 0.05
     "cylinder_128x64x64": dict(kind="cylinder", l=128, m=64, n=64, radius=8, centre=(32, 32),
-                               seed=2026, tau=0.6, u_in=0.05, diameter=16.0),
+                               seed=2026, tau=0.6, u_in=0.05, diameter=16.0, chain=()),
     # cylinder R = 12 with 8 helical 3 x 3 ridges at 45 degrees, tau 0.52, u_in 0.08
     "ridged_256x128x128": dict(kind="ridged", l=256, m=128, n=128, radius=12, centre=(64, 64),
-                               seed=2026, tau=0.52, u_in=0.08, ridge_width=3, diameter=30.0),
+                               seed=2026, tau=0.52, u_in=0.08, ridge_width=3, diameter=30.0,
+                               chain=((0.6, 1000), (0.55, 500))),
     # reticular sponge: hollow tube R = 48, wall 3, strut cell 16, tau 0.505, u_in 0.06
     "sponge_512x256x256": dict(l=512, m=256, n=256, radius=48, centre=(128, 128), cell=16,
                                thickness=3.0, seed=2026, tau=0.505, u_in=0.06, ridges=0,
-                               diameter=99.0),
+                               diameter=99.0, chain=((0.6, 1000), (0.55, 500), (0.52, 500))),
     # capacity run: the same generator scaled by 3 (R = 144, wall 9, cell 48) with 8 helical
     # 9 x 9 ridges on the outer surface, tau 0.502, u_in 0.05; 906 M sites
     "sponge_1536x768x768": dict(l=1536, m=768, n=768, radius=144, centre=(384, 384), cell=48,
                                 thickness=9.0, seed=2026, tau=0.502, u_in=0.05, ridges=8,
-                                ridge_width=9, diameter=315.0),
+                                ridge_width=9, diameter=315.0,
+                                chain=((0.6, 1000), (0.55, 500), (0.52, 500), (0.505, 500))),
 }
+
+# Start-up: plain BGK at tau close to 1/2 does not survive the impulsive start u = u_in everywhere
+# (the FP64 run of the 512 x 256 x 256 sponge at tau = 0.505 diverges after about 170 steps).
+# Like the reference's production runs, which restart each lower-viscosity run from the dump of
+# the previous one (Re 100 -> 500 -> ... -> 2000), every workload first runs a short chain of
+# higher-viscosity stages ``chain = ((tau, steps), ...)``, then switches to its own tau.
 
 
 def workload_mask(wl, nz=None):
@@ -77,6 +85,10 @@ def build_lattice(wl, storage, mask):
     lat.set_mask(mask)
     # u = (u_in, +-1e-4 seeded noise, 0) at every site, generated on the device
     lat.init_equilibrium(wl["u_in"], 0.0, 0.0, 1e-4, wl["seed"])
+    for tau, steps in wl.get("chain", ()):
+        lat.set_svisc(svisc_from_tau(tau))
+        lat.step(steps)
+    lat.set_svisc(svisc_from_tau(wl["tau"]))
     return lat
 
 

@@ -325,6 +325,14 @@ static int do_steps(eup_handle *h, int32_t nsteps)
     return 0;
 }
 
+int eup_set_omega(eup_handle *h, double omega)
+{
+    if (!h) return fail(nullptr, EUP_EINVAL, "eup_set_omega: null handle");
+    if (!(omega > 0.0 && omega < 2.0)) return fail(h, EUP_EINVAL, "eup_set_omega: omega must lie in (0, 2)");
+    h->cfg.omega = omega;
+    return 0;
+}
+
 int eup_step(eup_handle *h, int32_t nsteps)
 {
     int rc = ready(h, "eup_step");

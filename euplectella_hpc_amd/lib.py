@@ -31,7 +31,7 @@ EXPORTS = ("eup_create", "eup_destroy", "eup_last_error", "eup_set_mask", "eup_s
            "eup_sync", "eup_drag_cv", "eup_drag_mem", "eup_probe", "eup_diagno", "eup_get_info",
            "eup_stream", "eup_abi_version", "eup_set_grey", "eup_set_grey_device",
            "eup_grey_force", "eup_grey_objective", "eup_grey_objective_device", "eup_membw",
-           "eup_init_equilibrium")
+           "eup_init_equilibrium", "eup_set_omega")
 
 
 class EupConfig(ctypes.Structure):
@@ -102,6 +102,7 @@ def load():
     L.eup_abi_version.argtypes = []
     L.eup_abi_version.restype = i32
     L.eup_membw.argtypes = [i32, ctypes.c_int64, i32, dp]
+    L.eup_set_omega.argtypes = [vp, ctypes.c_double]
     L.eup_init_equilibrium.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_uint64]
     L.eup_set_grey.argtypes = [vp, dp]
@@ -198,6 +199,12 @@ class Lattice:
         if p.shape != (19, n + 2, m + 2, l + 2):
             raise ValueError("pops must have shape (19, n+2, m+2, l+2)")
         self._check(self.L.eup_set_state(self.h, _dptr(p)))
+
+    def set_svisc(self, svisc):
+        """Continue with a new viscosity (the reference's restart with a new deck svisc)."""
+        self.svisc = float(svisc)
+        self.omega = omega_from_svisc(svisc)
+        self._check(self.L.eup_set_omega(self.h, self.omega))
 
     def init_equilibrium(self, ux=0.0, uy=0.0, uz=0.0, noise=0.0, seed=0):
         """Equilibrium start on the device: u = (ux, uy + noise*U[-1,1), uz) per site."""

@@ -128,6 +128,11 @@ int eup_get_state(eup_handle *h, double *a);
 
 /* --- time stepping ------------------------------------------------------------------------- */
 
+/* New relaxation frequency for the following steps: the reference reaches low viscosity by
+ * restarting from the dump of a higher-viscosity run with a new svisc in the deck (the Re chain
+ * of CODE/RUN/README:15-21, omega recomputed by hencol.F90:30); this keeps the state on the GPU. */
+int eup_set_omega(eup_handle *h, double omega);
+
 /* nsteps iterations of `boundaries; propagation; collision` (bgk3d.F90:128-138). */
 int eup_step(eup_handle *h, int32_t nsteps);
 
