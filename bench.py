@@ -75,7 +75,7 @@ def parse_storage(spec):
     return (spec[:-1], True) if spec in ("f16s", "bf16s") else (spec, False)
 
 
-def build_lattice(wl, storage, mask):
+def build_lattice(wl, storage, mask, chain=True):
     from euplectella_hpc_amd.lib import Lattice, svisc_from_tau
 
     l, m, n = wl["l"], wl["m"], wl["n"]
@@ -85,15 +85,15 @@ def build_lattice(wl, storage, mask):
     lat.set_mask(mask)
     # u = (u_in, +-1e-4 seeded noise, 0) at every site, generated on the device
     lat.init_equilibrium(wl["u_in"], 0.0, 0.0, 1e-4, wl["seed"])
-    for tau, steps in wl.get("chain", ()):
+    for tau, steps in (wl.get("chain", ()) if chain else ()):
         lat.set_svisc(svisc_from_tau(tau))
         lat.step(steps)
     lat.set_svisc(svisc_from_tau(wl["tau"]))
     return lat
 
 
-def run_one(wl, storage, mask, steps, warmup):
-    lat = build_lattice(wl, storage, mask)
+def run_one(wl, storage, mask, steps, warmup, chain=True):
+    lat = build_lattice(wl, storage, mask, chain)
     try:
         info = lat.info()
         lat.step(warmup)
@@ -127,6 +127,8 @@ def run_one(wl, storage, mask, steps, warmup):
 
 
 def _torch_sync():
+    if os.environ.get("EUP_NO_TORCH"):
+        return
     try:
         import torch
         if torch.cuda.is_available():
@@ -241,6 +243,8 @@ def main(argv=None):
     ap.add_argument("--accuracy", type=int, default=0, metavar="STEPS",
                     help="only run the C_D / probe accuracy study of f32, f16, bf16 vs f64")
     ap.add_argument("--every", type=int, default=100, help="drag sampling interval (--accuracy)")
+    ap.add_argument("--no-chain", action="store_true",
+                    help="skip the higher-viscosity start-up chain (short profiling runs)")
     args = ap.parse_args(argv)
     if args.accuracy:
         r = accuracy(WORKLOADS[args.workload], ["f32", "f16", "f16s", "bf16", "bf16s"], args.accuracy,
@@ -261,7 +265,7 @@ def main(argv=None):
 
     wl = WORKLOADS[args.workload]
     mask = workload_mask(wl)
-    prim = run_one(wl, args.storage, mask, args.steps, args.warmup)
+    prim = run_one(wl, args.storage, mask, args.steps, args.warmup, not args.no_chain)
     if not prim["finite"]:
         print(f"bench.py: the {args.storage} state is not finite after the timed steps",
               file=sys.stderr)
@@ -270,7 +274,7 @@ def main(argv=None):
           f"{prim['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     sec = None
     if args.secondary and args.secondary != args.storage:
-        sec = run_one(wl, args.secondary, mask, args.steps, args.warmup)
+        sec = run_one(wl, args.secondary, mask, args.steps, args.warmup, not args.no_chain)
         print(f"[bench] {args.secondary}: {sec['mlups']:.0f} MLUPS, kernel {sec['kernel_ms']:.3f} ms",
               file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(wl)

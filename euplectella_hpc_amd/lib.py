@@ -72,10 +72,11 @@ def load():
     # One HIP runtime per process: PyTorch-ROCm bundles its own libamdhip64.so.7.  Importing torch
     # first makes the dynamic linker bind libeup's libamdhip64.so.7 to that already-loaded copy,
     # so library kernels and torch tensors share one runtime (device pointers, streams).
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+    if not os.environ.get("EUP_NO_TORCH"):
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     L = ctypes.CDLL(LIB_PATH)
     vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
     i32 = ctypes.c_int32
