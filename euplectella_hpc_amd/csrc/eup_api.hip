@@ -99,7 +99,7 @@ static void release(Handle *h)
 {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
-    void *ptrs[] = {h->buf[0], h->buf[1], h->staging, h->obs, h->solid, h->solid_list, h->bb_tmp,
+    void *ptrs[] = {h->alloc[0], h->alloc[1], h->staging, h->obs, h->solid, h->solid_list, h->bb_tmp,
                     h->tile_code, h->link_mask, h->mixed_list, h->outlet[0], h->outlet[1], h->diag,
                     h->site19, h->grey};
     for (void *p : ptrs)
@@ -145,15 +145,24 @@ int eup_create(const eup_config *cfg, eup_handle **out)
     h->elem = storage_bytes(cfg->storage);
     const int64_t per = cfg->mode == EUP_MODE_PARITY ? g.ghosted : g.sites;
     h->stride = (per + 255) / 256 * 256;
+    h->tiled = cfg->mode == EUP_MODE_PHYSICS;
     int rc = 0;
     if (hip_check(h, hipSetDevice(cfg->device), "hipSetDevice") ||
         hip_check(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate") ||
         hip_check(h, hipEventCreate(&h->ev0), "hipEventCreate") ||
         hip_check(h, hipEventCreate(&h->ev1), "hipEventCreate"))
         rc = EUP_EHIP;
-    const size_t state = h->elem * (size_t)h->stride * 19;
-    for (int b = 0; b < 2 && !rc; ++b)
-        if (hip_check(h, hipMalloc(&h->buf[b], state), "hipMalloc populations")) rc = EUP_EHIP;
+    // physics mode: tile-major with one guard tile on either side; other modes: SoA
+    const size_t state = h->tiled ? h->elem * (size_t)kTileElems * (size_t)(g.tiles + 2)
+                                  : h->elem * (size_t)h->stride * 19;
+    const size_t lead = h->tiled ? h->elem * (size_t)kTileElems : 0;
+    for (int b = 0; b < 2 && !rc; ++b) {
+        if (hip_check(h, hipMalloc(&h->alloc[b], state), "hipMalloc populations")) rc = EUP_EHIP;
+        else h->buf[b] = static_cast<char *>(h->alloc[b]) + lead;
+    }
+    // defined contents in the guard tiles and the padding lanes of partial tiles
+    for (int b = 0; b < 2 && !rc && h->tiled; ++b)
+        if (hip_check(h, hipMemset(h->alloc[b], 0, state), "hipMemset populations")) rc = EUP_EHIP;
     if (!rc && (hip_check(h, hipMalloc((void **)&h->staging, sizeof(double) * g.ghosted), "hipMalloc staging") ||
                 hip_check(h, hipMalloc((void **)&h->obs, sizeof(int32_t) * g.sites), "hipMalloc obs") ||
                 hip_check(h, hipMalloc((void **)&h->solid, g.sites), "hipMalloc solid")))

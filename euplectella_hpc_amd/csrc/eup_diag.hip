@@ -29,12 +29,13 @@ struct ParityAcc {
 
 template <typename S> struct PhysAcc {
     using value_type = S;
-    const S *p[19];
+    const S *base;
+    PopIndex ix;
     DevRef ref;
     int l, m, n;
     __device__ __forceinline__ double operator()(int q, int i, int j, int k) const
     {
-        return (double)Store<S>::load(p[q][(int64_t)(i - 1) + (int64_t)l * ((int64_t)(j - 1) + (int64_t)m * (k - 1))], ref.r[q]);
+        return (double)Store<S>::load(base[ix(q, i - 1, j - 1, k - 1)], ref.r[q]);
     }
 };
 
@@ -261,8 +262,8 @@ ParityAcc parity_acc(Handle *h)
 template <typename S> PhysAcc<S> phys_acc(Handle *h)
 {
     PhysAcc<S> a;
-    const S *base = static_cast<const S *>(h->buf[h->cur]);
-    for (int q = 0; q < 19; ++q) a.p[q] = base + (int64_t)q * h->stride;
+    a.base = static_cast<const S *>(h->buf[h->cur]);
+    a.ix = pop_index(h);
     a.ref = dev_ref(h);
     a.l = h->g.l;
     a.m = h->g.m;

@@ -13,6 +13,7 @@ namespace eup {
 
 constexpr int kTile = 64;     // sites per tile = one wavefront along x
 constexpr int kBlock = 256;   // threads per block in the site kernels (4 tiles)
+constexpr int kTileElems = 19 * kTile; // population values per tile in the physics layout
 
 // tile codes of the physics layout
 constexpr int32_t kTileFluid = -1; // no solid site in the tile or adjacent to it
@@ -33,8 +34,10 @@ struct Handle {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t elem = 8;              // bytes per stored population value
-    int64_t stride = 0;           // elements between populations in a state buffer
-    void *buf[2] = {nullptr, nullptr}; // ping-pong population buffers
+    int64_t stride = 0;           // elements between populations in an SoA state buffer
+    bool tiled = false;           // physics mode: tile-major layout (PopIndex)
+    void *alloc[2] = {nullptr, nullptr}; // device allocations of the two state buffers
+    void *buf[2] = {nullptr, nullptr}; // ping-pong population buffers (past the leading guard)
     int cur = 0;                  // buf[cur] holds the current state
     double *staging = nullptr;    // ghosted FP64 staging array (one population)
     bool mask_set = false;
@@ -93,6 +96,38 @@ int diag_site(Handle *h, int i, int j, int k, double out[19]);
 int diag_drag_cv(Handle *h, const int32_t box[6], double *fx, double *fy);
 int diag_drag_mem(Handle *h, double F[3]);
 int diag_diagno(Handle *h, double out[5]);
+
+// Element index of population q of site (i, j, k) (0-based) in a dense state buffer.
+//  * SoA (grey mode): q * stride + i + l * (j + m * k).
+//  * tile-major (physics mode): the lattice is cut into tiles of 64 consecutive sites along x
+//    (tpr = ceil(l / 64) per row, the last one padded); tile t = i / 64 + tpr * (j + m * k) holds
+//    its populations as 19 runs of 64 values, so population q of the site sits at
+//    (t * 19 + q) * 64 + i % 64.  One guard tile precedes tile 0 and one follows the last, so that
+//    a pull shifted by one site in x from the first or last tile stays inside the allocation.
+struct PopIndex {
+    int64_t stride;
+    int l, m, tpr, tiled;
+    __host__ __device__ __forceinline__ int64_t operator()(int q, int i, int j, int k) const
+    {
+        const int64_t row = (int64_t)j + (int64_t)m * k;
+        if (tiled) {
+            const int64_t t = (int64_t)(i >> 6) + (int64_t)tpr * row;
+            return ((t * 19 + q) << 6) + (i & 63);
+        }
+        return (int64_t)q * stride + i + (int64_t)l * row;
+    }
+};
+
+inline PopIndex pop_index(const Handle *h)
+{
+    PopIndex p;
+    p.stride = h->stride;
+    p.l = h->g.l;
+    p.m = h->g.m;
+    p.tpr = h->g.tiles_per_row;
+    p.tiled = h->tiled ? 1 : 0;
+    return p;
+}
 
 // deviation reference of the handle: w_q, or f_eq(rho = 1, u = (u_inflow, 0, 0)) with
 // EUP_FLAG_SHIFTED_DEVIATION

@@ -5,10 +5,13 @@
 // no stale seam and no frozen edge ghost), bounce-back taken from the site's own opposite
 // population (bcond.obs.F90:56-73 with fresh neighbours), and solid sites left untouched.
 //
-// Layout in HBM: populations are SoA, 19 arrays of l*m*n sites each (i fastest, no ghost layer),
-// two buffers (A current, B next).  The lattice is cut into tiles of 64 consecutive sites along x;
-// one wavefront updates one tile, so every load and store of a population is a contiguous 64-lane
-// access (256 B in FP32, 128 B in FP16).  Per tile a 4-byte code says whether the tile is pure
+// Layout in HBM: two buffers (A current, B next), tile-major.  The lattice is cut into tiles of 64
+// consecutive sites along x; a tile stores its 19 populations as 19 contiguous runs of 64 values
+// (PopIndex), one guard tile before and after the lattice.  One wavefront updates one tile, so
+// every load and store of a population is a contiguous 64-lane access (256 B in FP32, 128 B in
+// FP16) whose address is a wave-uniform row pointer (scalar registers), one of three lane offsets
+// (x shift -1, 0, +1) and the constant population offset Q * 64 values.  Per tile a 4-byte code
+// says whether the tile is pure
 // fluid (no solid site in or next to it: no mask traffic at all), all solid (skipped: no loads, no
 // stores), or mixed (the 64 per-site link masks are read: bit q set = the source of population q
 // is solid, bit 31 = the site itself is solid).  The codes are built once per mask with a
@@ -27,12 +30,12 @@ namespace {
 
 template <typename S> struct XArgs {
     using R = typename Store<S>::compute;
-    const S *__restrict__ A[19];
-    S *__restrict__ B[19];
-    const int32_t *__restrict__ tile_code;
-    const uint32_t *__restrict__ link;
-    const R *__restrict__ out_cur; // 5 x (m*n)
-    R *__restrict__ out_next;
+    const S *A;  // tile 0 of the current state (tile-major layout)
+    S *B;        // tile 0 of the next state
+    const int32_t *tile_code;
+    const uint32_t *link;
+    const R *out_cur; // 5 x (m*n)
+    R *out_next;
     R inl[5];
     DevRef ref;
     R omega, fgrad;
@@ -45,59 +48,86 @@ template <int... Qs> struct QSeq {};
 using Q18 = QSeq<0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17>;
 using Q19 = QSeq<0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18>;
 
-struct Rows {
-    // site offsets of the nine source rows, indexed [dy+1][dz+1]
-    uint32_t off[3][3];
+// Addressing of one wavefront's tile: the tile of each of the nine source rows (wave-uniform
+// byte pointers, indexed [dy+1][dz+1] for the row j+dy, k+dz with periodic wrap) and the lane byte
+// offsets of the three x shifts.  A population pulled along c with c_x = +1 comes from lane - 1,
+// which for lane 0 is lane 63 of the previous tile: it is addressed from the row pointer minus one
+// tile so that every lane offset is non-negative.  For c_x = -1 lane 63 reads lane 0 of the next
+// tile.
+struct Wave {
+    const char *row[3][3];
+    uint32_t o0, op, om;
     int jj[3], kk[3];
+    int ti, i;
 };
 
+template <typename S>
+__device__ __forceinline__ void wave_setup(const XArgs<S> &a, Wave &w, int ti, int j, int k, int lane)
+{
+    constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
+    const int m = a.m, n = a.n;
+    w.ti = ti;
+    w.i = ti * kTile + lane;
+    w.jj[0] = (j == 0) ? m - 1 : j - 1;
+    w.jj[1] = j;
+    w.jj[2] = (j == m - 1) ? 0 : j + 1;
+    w.kk[0] = (k == 0) ? n - 1 : k - 1;
+    w.kk[1] = k;
+    w.kk[2] = (k == n - 1) ? 0 : k + 1;
+    const char *A = reinterpret_cast<const char *>(a.A);
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+            w.row[dy][dz] = A + ((int64_t)ti + (int64_t)a.tpr * ((int64_t)w.jj[dy] + (int64_t)m * w.kk[dz])) * TB;
+    w.o0 = (uint32_t)lane * (uint32_t)sizeof(S);
+    w.op = (lane == 0) ? (uint32_t)(kTile - 1) * (uint32_t)sizeof(S) : (uint32_t)TB + (uint32_t)(lane - 1) * (uint32_t)sizeof(S);
+    w.om = (lane == kTile - 1) ? (uint32_t)TB : (uint32_t)(lane + 1) * (uint32_t)sizeof(S);
+}
+
+template <typename S>
+__device__ __forceinline__ const S *at(const char *p, int64_t bytes, uint32_t off)
+{
+    return reinterpret_cast<const S *>(p + bytes + (uint64_t)off);
+}
+
 template <typename S, bool XOPEN, bool MIXED, int Q>
-__device__ __forceinline__ void pull(const XArgs<S> &a, const Rows &r, int i, uint32_t s,
-                                     uint32_t lm, typename Store<S>::compute *x)
+__device__ __forceinline__ void pull(const XArgs<S> &a, const Wave &w, uint32_t lm, typename Store<S>::compute *x)
 {
     using R = typename Store<S>::compute;
     constexpr int cx = Dir<Q>::cx, cy = Dir<Q>::cy, cz = Dir<Q>::cz, op = Dir<Q>::opp;
-    // source site = x - c_Q
-    int si = i - cx;
-    if constexpr (!XOPEN) {
-        if constexpr (cx == 1) si = (i == 0) ? a.l - 1 : si;
-        if constexpr (cx == -1) si = (i == a.l - 1) ? 0 : si;
-    } else {
-        // the inlet / outlet lanes replace the value below; keep their load inside the row
-        if constexpr (cx == 1) si = (i == 0) ? 0 : si;
-        if constexpr (cx == -1) si = (i == a.l - 1) ? a.l - 1 : si;
+    constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
+    constexpr int64_t QB = (int64_t)Q * kTile * sizeof(S);
+    const char *rb = w.row[1 - cy][1 - cz]; // source row j - c_y, k - c_z
+    R v;
+    if constexpr (cx == 1) v = Store<S>::load(*at<S>(rb, QB - TB, w.op), a.ref.r[Q]);
+    else if constexpr (cx == 0) v = Store<S>::load(*at<S>(rb, QB, w.o0), a.ref.r[Q]);
+    else v = Store<S>::load(*at<S>(rb, QB, w.om), a.ref.r[Q]);
+    // x boundaries: only the first and last tile of a row hold such sites (wave-uniform tests)
+    if constexpr (cx == 1) {
+        if (w.ti == 0 && w.i == 0) {
+            if constexpr (XOPEN) v = a.inl[Q];
+            else v = Store<S>::load(*at<S>(rb, QB + (int64_t)(a.tpr - 1) * TB, (uint32_t)((a.l - 1) & 63) * (uint32_t)sizeof(S)), a.ref.r[Q]);
+        }
     }
-    const uint32_t src = r.off[1 - cy][1 - cz] + (uint32_t)si;
-    uint32_t addr_q = src;
-    const S *base = a.A[Q];
-    float refq = a.ref.r[Q];
+    if constexpr (cx == -1) {
+        if (w.ti == a.tpr - 1 && w.i == a.l - 1) {
+            if constexpr (XOPEN) v = a.out_cur[(Q - 9) * (size_t)(a.m * a.n) + w.jj[1 - cy] + (size_t)a.m * w.kk[1 - cz]];
+            else v = Store<S>::load(*at<S>(rb, QB - (int64_t)w.ti * TB, 0u), a.ref.r[Q]);
+        }
+    }
     if constexpr (MIXED) {
-        if ((lm >> Q) & 1u) { // the source is solid: bounce back the site's own population
-            base = a.A[op];
-            addr_q = s;
-            refq = a.ref.r[op];
-        }
-    }
-    R v = Store<S>::load(base[addr_q], refq);
-    if constexpr (XOPEN) {
-        if constexpr (cx == 1) {
-            if (i == 0) v = a.inl[Q];
-        }
-        if constexpr (cx == -1) {
-            if (i == a.l - 1) {
-                const int jj = r.jj[1 - cy], kk = r.kk[1 - cz];
-                v = a.out_cur[(Q - 9) * (size_t)(a.m * a.n) + jj + a.m * kk];
-            }
-        }
+        // the source is solid: bounce back the site's own opposite population
+        if ((lm >> Q) & 1u) v = Store<S>::load(*at<S>(w.row[1][1], (int64_t)op * kTile * sizeof(S), w.o0), a.ref.r[op]);
     }
     x[Q] = v;
 }
 
 template <typename S, bool XOPEN, bool MIXED, int... Qs>
-__device__ __forceinline__ void pull_all(QSeq<Qs...>, const XArgs<S> &a, const Rows &r, int i,
-                                         uint32_t s, uint32_t lm, typename Store<S>::compute *x)
+__device__ __forceinline__ void pull_all(QSeq<Qs...>, const XArgs<S> &a, const Wave &w, uint32_t lm,
+                                         typename Store<S>::compute *x)
 {
-    (pull<S, XOPEN, MIXED, Qs>(a, r, i, s, lm, x), ...);
+    (pull<S, XOPEN, MIXED, Qs>(a, w, lm, x), ...);
 }
 
 #ifndef EUP_NT_STORE
@@ -117,59 +147,61 @@ template <typename S> __device__ __forceinline__ void store_val(S *p, S v)
 }
 
 template <typename S, int Q>
-__device__ __forceinline__ void put(const XArgs<S> &a, uint32_t s, const typename Store<S>::compute *x)
+__device__ __forceinline__ void put(const XArgs<S> &a, char *tb, uint32_t o0, const typename Store<S>::compute *x)
 {
-    store_val<S>(&a.B[Q][s], Store<S>::store(x[Q], a.ref.r[Q]));
+    S *p = reinterpret_cast<S *>(tb + (int64_t)Q * kTile * sizeof(S) + (uint64_t)o0);
+    store_val<S>(p, Store<S>::store(x[Q], a.ref.r[Q]));
 }
 
 template <typename S, int... Qs>
-__device__ __forceinline__ void put_all(QSeq<Qs...>, const XArgs<S> &a, uint32_t s,
+__device__ __forceinline__ void put_all(QSeq<Qs...>, const XArgs<S> &a, char *tb, uint32_t o0,
                                         const typename Store<S>::compute *x)
 {
-    (put<S, Qs>(a, s, x), ...);
+    (put<S, Qs>(a, tb, o0, x), ...);
 }
 
-template <typename S, bool XOPEN, bool MIXED>
-__device__ __forceinline__ void update_site(const XArgs<S> &a, int i, int j, int k, uint32_t lm)
+// Update NT adjacent tiles t .. t+NT-1 of one row, one site per lane in each.  MIXED (NT = 1 only)
+// applies the link mask lm of the lane's site: bounce-back from solid sources, solid sites frozen.
+template <typename S, bool XOPEN, bool MIXED, int NT>
+__device__ __forceinline__ void update_tiles(const XArgs<S> &a, uint32_t t, uint32_t lm)
 {
     using R = typename Store<S>::compute;
+    constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
+    const uint32_t row = t / (uint32_t)a.tpr;
+    const int ti = (int)(t - row * (uint32_t)a.tpr);
+    const int lane = (int)(threadIdx.x & 63);
     if constexpr (MIXED) {
         if (lm & 0x80000000u) return; // solid site: frozen
     }
-    const int m = a.m, n = a.n, l = a.l;
-    Rows r;
-    r.jj[0] = (j == 0) ? m - 1 : j - 1; // source row j - 1   (dy = -1)
-    r.jj[1] = j;
-    r.jj[2] = (j == m - 1) ? 0 : j + 1;
-    r.kk[0] = (k == 0) ? n - 1 : k - 1;
-    r.kk[1] = k;
-    r.kk[2] = (k == n - 1) ? 0 : k + 1;
+    const int j = (int)(row % (uint32_t)a.m), k = (int)(row / (uint32_t)a.m);
+    Wave w[NT];
+    R x[NT][19];
 #pragma unroll
-    for (int dy = 0; dy < 3; ++dy)
+    for (int u = 0; u < NT; ++u) {
+        wave_setup<S>(a, w[u], ti + u, j, k, lane);
+        pull_all<S, XOPEN, MIXED>(Q18{}, a, w[u], lm, x[u]);
+        x[u][18] = Store<S>::load(*at<S>(w[u].row[1][1], 18 * kTile * (int64_t)sizeof(S), w[u].o0), a.ref.r[18]);
+    }
+    char *tb = reinterpret_cast<char *>(a.B) + (int64_t)t * TB;
 #pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-            r.off[dy][dz] = (uint32_t)l * (uint32_t)(r.jj[dy] + m * r.kk[dz]);
-    const uint32_t s = r.off[1][1] + (uint32_t)i;
-    R x[19];
-    pull_all<S, XOPEN, MIXED>(Q18{}, a, r, i, s, lm, x);
-    x[18] = Store<S>::load(a.A[18][s], a.ref.r[18]);
-    collide<R>(x, x, a.omega, a.fgrad);
-    put_all<S>(Q19{}, a, s, x);
-    if constexpr (XOPEN) {
-        if (i == l - 1) {
-            R o[5];
-            outlet_eq<R>(x, o);
-            const size_t mn = (size_t)m * n;
+    for (int u = 0; u < NT; ++u) {
+        if (w[u].i >= a.l) continue; // padding lanes of a partial last tile
+        collide<R>(x[u], x[u], a.omega, a.fgrad);
+        put_all<S>(Q19{}, a, tb + u * TB, w[u].o0, x[u]);
+        if constexpr (XOPEN) {
+            if (w[u].i == a.l - 1) {
+                R o[5];
+                outlet_eq<R>(x[u], o);
+                const size_t mn = (size_t)a.m * a.n;
 #pragma unroll
-            for (int u = 0; u < 5; ++u) a.out_next[u * mn + j + (size_t)m * k] = o[u];
+                for (int v = 0; v < 5; ++v) a.out_next[v * mn + j + (size_t)a.m * k] = o[v];
+            }
         }
     }
 }
 
 // Tiles per wavefront: NT adjacent pure-fluid tiles of one row are updated together (NT
-// independent loads per population per lane in flight); other tiles one at a time.  Two tiles
-// per wave pay off for the 2-byte storages, whose single-tile loads move only 128 B per
-// instruction; 4- and 8-byte storages run one tile per wave.
+// independent loads per population per lane in flight); other tiles one at a time.
 template <typename S> struct TilesPerWave {
     static constexpr int value = sizeof(S) == 2 ? 2 : 1;
 };
@@ -179,60 +211,15 @@ template <typename S> struct TilesPerWave {
 #define EUP_TPW_OF(S) TilesPerWave<S>::value
 #endif
 
-template <typename S, bool XOPEN, int NT>
-__device__ __forceinline__ void update_multi(const XArgs<S> &a, int i, int j, int k)
-{
-    using R = typename Store<S>::compute;
-    const int m = a.m, n = a.n, l = a.l;
-    Rows r;
-    r.jj[0] = (j == 0) ? m - 1 : j - 1;
-    r.jj[1] = j;
-    r.jj[2] = (j == m - 1) ? 0 : j + 1;
-    r.kk[0] = (k == 0) ? n - 1 : k - 1;
-    r.kk[1] = k;
-    r.kk[2] = (k == n - 1) ? 0 : k + 1;
-#pragma unroll
-    for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-            r.off[dy][dz] = (uint32_t)l * (uint32_t)(r.jj[dy] + m * r.kk[dz]);
-    const uint32_t s0 = r.off[1][1] + (uint32_t)i;
-    R x[NT][19];
-#pragma unroll
-    for (int u = 0; u < NT; ++u) {
-        pull_all<S, XOPEN, false>(Q18{}, a, r, i + u * kTile, s0 + u * kTile, 0u, x[u]);
-        x[u][18] = Store<S>::load(a.A[18][s0 + u * kTile], a.ref.r[18]);
-    }
-#pragma unroll
-    for (int u = 0; u < NT; ++u) {
-        collide<R>(x[u], x[u], a.omega, a.fgrad);
-        put_all<S>(Q19{}, a, s0 + u * kTile, x[u]);
-    }
-    if constexpr (XOPEN) {
-        if (i + (NT - 1) * kTile == l - 1) {
-            R o[5];
-            outlet_eq<R>(x[NT - 1], o);
-            const size_t mn = (size_t)m * n;
-#pragma unroll
-            for (int u = 0; u < 5; ++u) a.out_next[u * mn + j + (size_t)m * k] = o[u];
-        }
-    }
-}
-
 template <typename S, bool XOPEN>
 __device__ __forceinline__ void do_tile(const XArgs<S> &a, uint32_t t, int32_t code)
 {
     if (code == kTileSolid) return;
-    const uint32_t row = t / (uint32_t)a.tpr;
-    const int i = (int)(t - row * (uint32_t)a.tpr) * kTile + (int)(threadIdx.x & 63);
-    if (i >= a.l) return;
-    const int j = (int)(row % (uint32_t)a.m);
-    const int k = (int)(row / (uint32_t)a.m);
     if (code == kTileFluid) {
-        update_site<S, XOPEN, false>(a, i, j, k, 0u);
+        update_tiles<S, XOPEN, false, 1>(a, t, 0u);
     } else {
         const uint32_t lm = a.link[(size_t)code * kTile + (threadIdx.x & 63)];
-        update_site<S, XOPEN, true>(a, i, j, k, lm);
+        update_tiles<S, XOPEN, true, 1>(a, t, lm);
     }
 }
 
@@ -251,17 +238,13 @@ __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
     if constexpr (kTilesPerWave == 1) {
         do_tile<S, XOPEN>(a, t0, c0);
     } else {
-        const uint32_t row = t0 / (uint32_t)a.tpr;
-        const uint32_t seg = t0 - row * (uint32_t)a.tpr;
-        bool all_fluid = c0 == kTileFluid && seg + kTilesPerWave <= (uint32_t)a.tpr &&
-                         (int)(seg + kTilesPerWave) * kTile <= a.l;
+        const uint32_t seg = t0 % (uint32_t)a.tpr;
+        bool all_fluid = c0 == kTileFluid && seg + kTilesPerWave <= (uint32_t)a.tpr;
 #pragma unroll
         for (int u = 1; u < kTilesPerWave; ++u)
             all_fluid = all_fluid && t0 + u < a.tiles && a.tile_code[t0 + u] == kTileFluid;
         if (__builtin_amdgcn_readfirstlane((int)all_fluid)) {
-            const int i = (int)seg * kTile + (int)(threadIdx.x & 63);
-            update_multi<S, XOPEN, kTilesPerWave>(a, i, (int)(row % (uint32_t)a.m),
-                                                  (int)(row / (uint32_t)a.m));
+            update_tiles<S, XOPEN, false, kTilesPerWave>(a, t0, 0u);
         } else {
             do_tile<S, XOPEN>(a, t0, c0);
 #pragma unroll
@@ -334,7 +317,7 @@ __global__ void k_compact_links(const int32_t *__restrict__ code, const uint32_t
 // ---------------------------------------------------------------------------------------------
 template <typename S>
 __global__ void k_from_staging(const double *__restrict__ stg, S *__restrict__ d0, S *__restrict__ d1,
-                               int l, int m, int n, int q, float ref)
+                               PopIndex P, int l, int m, int n, int q, float ref)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t N = (int64_t)l * m * n;
@@ -345,13 +328,14 @@ __global__ void k_from_staging(const double *__restrict__ stg, S *__restrict__ d
     const double v = stg[(int64_t)(i + 1) + (int64_t)(l + 2) * ((int64_t)(j + 1) + (int64_t)(m + 2) * (k + 1))];
     using R = typename Store<S>::compute;
     const S sv = Store<S>::store((R)v, ref);
-    d0[t] = sv;
-    d1[t] = sv;
+    const int64_t e = P(q, i, j, k);
+    d0[e] = sv;
+    d1[e] = sv;
 }
 
 template <typename S>
-__global__ void k_to_staging(const S *__restrict__ src, double *__restrict__ stg, int l, int m, int n,
-                             int q, float ref)
+__global__ void k_to_staging(const S *__restrict__ src, double *__restrict__ stg, PopIndex P, int l, int m,
+                             int n, int q, float ref)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t N = (int64_t)l * m * n;
@@ -360,7 +344,7 @@ __global__ void k_to_staging(const S *__restrict__ src, double *__restrict__ stg
     const int64_t r = t / l;
     const int j = (int)(r % m), k = (int)(r / m);
     stg[(int64_t)(i + 1) + (int64_t)(l + 2) * ((int64_t)(j + 1) + (int64_t)(m + 2) * (k + 1))] =
-        (double)Store<S>::load(src[t], ref);
+        (double)Store<S>::load(src[P(q, i, j, k)], ref);
 }
 
 // counter-based uniform in [-1, 1) (splitmix64 finaliser)
@@ -373,13 +357,17 @@ __device__ __forceinline__ double hash_uniform(uint64_t seed, uint64_t i)
     return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-// equilibrium start; ghosted = 1: parity layout (l+2)(m+2)(n+2), else dense l*m*n
+// equilibrium start over nsites sites; site t is (t % l, (t / l) % m, t / (l m)) of the layout P
+// (the parity layout passes its ghosted sites as one row: l = nsites, m = 1)
 template <typename S>
-__global__ void k_init_eq(S *base, int64_t stride, int64_t nsites, double ux, double uy0, double uz,
+__global__ void k_init_eq(S *base, PopIndex P, int l, int m, int64_t nsites, double ux, double uy0, double uz,
                           double noise, uint64_t seed, DevRef ref)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsites) return;
+    const int i = (int)(t % l);
+    const int64_t r = t / l;
+    const int j = (int)(r % m), k = (int)(r / m);
     const double uy = uy0 + noise * hash_uniform(seed, (uint64_t)t);
     const double usq = ux * ux + uy * uy + uz * uz;
     const int cx[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
@@ -390,7 +378,7 @@ __global__ void k_init_eq(S *base, int64_t stride, int64_t nsites, double ux, do
         const double cu = cx[q] * ux + cy[q] * uy + cz[q] * uz;
         const double f = weight<double>(q) * (1.0 + (3.0 * cu + 1.5 * (3.0 * cu * cu - usq)));
         using R = typename Store<S>::compute;
-        base[q * stride + t] = Store<S>::store((R)f, ref.r[q]);
+        base[P(q, i, j, k)] = Store<S>::store((R)f, ref.r[q]);
     }
 }
 
@@ -402,10 +390,16 @@ __global__ void k_outlet_refresh(XArgs<S> a, typename Store<S>::compute *o0,
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t mn = (int64_t)a.m * a.n;
     if (t >= mn) return;
-    const uint32_t s = (uint32_t)(a.l - 1) + (uint32_t)a.l * (uint32_t)t;
+    const int j = (int)(t % a.m), k = (int)(t / a.m);
+    PopIndex P;
+    P.stride = 0;
+    P.l = a.l;
+    P.m = a.m;
+    P.tpr = a.tpr;
+    P.tiled = 1;
     R x[19], o[5];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) x[q] = Store<S>::load(a.A[q][s], a.ref.r[q]);
+    for (int q = 0; q < 19; ++q) x[q] = Store<S>::load(a.A[P(q, a.l - 1, j, k)], a.ref.r[q]);
     outlet_eq<R>(x, o);
 #pragma unroll
     for (int u = 0; u < 5; ++u) {
@@ -418,12 +412,8 @@ template <typename S> XArgs<S> make_args(Handle *h)
 {
     using R = typename Store<S>::compute;
     XArgs<S> a;
-    S *src = static_cast<S *>(h->buf[h->cur]);
-    S *dst = static_cast<S *>(h->buf[h->cur ^ 1]);
-    for (int q = 0; q < 19; ++q) {
-        a.A[q] = src + (int64_t)q * h->stride;
-        a.B[q] = dst + (int64_t)q * h->stride;
-    }
+    a.A = static_cast<const S *>(h->buf[h->cur]);
+    a.B = static_cast<S *>(h->buf[h->cur ^ 1]);
     a.tile_code = h->tile_code;
     a.link = h->link_mask;
     a.out_cur = static_cast<const R *>(h->outlet[h->cur]);
@@ -472,20 +462,22 @@ template <typename S> int refresh_t(Handle *h)
 
 template <typename S> int set_t(Handle *h, int q)
 {
-    S *d0 = static_cast<S *>(h->buf[0]) + (int64_t)(q - 1) * h->stride;
-    S *d1 = static_cast<S *>(h->buf[1]) + (int64_t)(q - 1) * h->stride;
+    S *d0 = static_cast<S *>(h->buf[0]);
+    S *d1 = static_cast<S *>(h->buf[1]);
     hipLaunchKernelGGL((k_from_staging<S>), dim3((unsigned)((h->g.sites + 255) / 256)), dim3(256), 0,
-                       h->stream, h->staging, d0, d1, h->g.l, h->g.m, h->g.n, q - 1, dev_ref(h).r[q - 1]);
+                       h->stream, h->staging, d0, d1, pop_index(h), h->g.l, h->g.m, h->g.n, q - 1,
+                       dev_ref(h).r[q - 1]);
     return hip_check(h, hipGetLastError(), "set pops") ? EUP_EHIP : 0;
 }
 
 template <typename S> int get_t(Handle *h, int q)
 {
-    const S *s = static_cast<const S *>(h->buf[h->cur]) + (int64_t)(q - 1) * h->stride;
+    const S *s = static_cast<const S *>(h->buf[h->cur]);
     if (hip_check(h, hipMemsetAsync(h->staging, 0, sizeof(double) * h->g.ghosted, h->stream), "memset"))
         return EUP_EHIP;
     hipLaunchKernelGGL((k_to_staging<S>), dim3((unsigned)((h->g.sites + 255) / 256)), dim3(256), 0,
-                       h->stream, s, h->staging, h->g.l, h->g.m, h->g.n, q - 1, dev_ref(h).r[q - 1]);
+                       h->stream, s, h->staging, pop_index(h), h->g.l, h->g.m, h->g.n, q - 1,
+                       dev_ref(h).r[q - 1]);
     return hip_check(h, hipGetLastError(), "get pops") ? EUP_EHIP : 0;
 }
 
@@ -533,11 +525,13 @@ int init_equilibrium(Handle *h, double ux, double uy, double uz, double noise, u
     const bool parity = h->cfg.mode == EUP_MODE_PARITY;
     const int64_t ns = parity ? h->g.ghosted : h->g.sites;
     const unsigned grid = (unsigned)((ns + 255) / 256);
+    const PopIndex P = pop_index(h);
+    const int il = parity ? (int)h->g.ghosted : h->g.l, im = parity ? 1 : h->g.m;
     for (int b = 0; b < 2; ++b) {
         int rc = dispatch(h, [&](auto *p) {
             using S = std::remove_pointer_t<decltype(p)>;
             hipLaunchKernelGGL((k_init_eq<S>), dim3(grid), dim3(256), 0, h->stream, static_cast<S *>(h->buf[b]),
-                               h->stride, ns, ux, uy, uz, noise, seed, dev_ref(h));
+                               P, il, im, ns, ux, uy, uz, noise, seed, dev_ref(h));
             return hip_check(h, hipGetLastError(), "init equilibrium") ? EUP_EHIP : 0;
         });
         if (rc) return rc;
