@@ -74,12 +74,16 @@ __device__ __forceinline__ void wave_setup(const XArgs<S> &a, Wave &w, int ti, i
     w.kk[0] = (k == 0) ? n - 1 : k - 1;
     w.kk[1] = k;
     w.kk[2] = (k == n - 1) ? 0 : k + 1;
+    // row tile indices fit in 32 bits (tiles < 2^32): one widening multiply per row pointer
     const char *A = reinterpret_cast<const char *>(a.A);
+    const uint32_t tpr = (uint32_t)a.tpr, tpm = tpr * (uint32_t)m;
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-            w.row[dy][dz] = A + ((int64_t)ti + (int64_t)a.tpr * ((int64_t)w.jj[dy] + (int64_t)m * w.kk[dz])) * TB;
+        for (int dz = 0; dz < 3; ++dz) {
+            const uint32_t rt = (uint32_t)ti + tpr * (uint32_t)w.jj[dy] + tpm * (uint32_t)w.kk[dz];
+            w.row[dy][dz] = A + (uint64_t)rt * (uint64_t)TB;
+        }
     w.o0 = (uint32_t)lane * (uint32_t)sizeof(S);
     w.op = (lane == 0) ? (uint32_t)(kTile - 1) * (uint32_t)sizeof(S) : (uint32_t)TB + (uint32_t)(lane - 1) * (uint32_t)sizeof(S);
     w.om = (lane == kTile - 1) ? (uint32_t)TB : (uint32_t)(lane + 1) * (uint32_t)sizeof(S);
@@ -91,43 +95,84 @@ __device__ __forceinline__ const S *at(const char *p, int64_t bytes, uint32_t of
     return reinterpret_cast<const S *>(p + bytes + (uint64_t)off);
 }
 
-template <typename S, bool XOPEN, bool MIXED, int Q>
+// Pull of population Q, first pass: the load from the interior source x - c_Q, or (MIXED, source
+// solid) the site's own opposite population.  All 19 loads are issued back to back; the sites
+// whose source lies outside the lattice in x are fixed afterwards (x_edges).
+template <typename S, bool MIXED, int Q>
 __device__ __forceinline__ void pull(const XArgs<S> &a, const Wave &w, uint32_t lm, typename Store<S>::compute *x)
 {
-    using R = typename Store<S>::compute;
     constexpr int cx = Dir<Q>::cx, cy = Dir<Q>::cy, cz = Dir<Q>::cz, op = Dir<Q>::opp;
     constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
     constexpr int64_t QB = (int64_t)Q * kTile * sizeof(S);
     const char *rb = w.row[1 - cy][1 - cz]; // source row j - c_y, k - c_z
-    R v;
-    if constexpr (cx == 1) v = Store<S>::load(*at<S>(rb, QB - TB, w.op), a.ref.r[Q]);
-    else if constexpr (cx == 0) v = Store<S>::load(*at<S>(rb, QB, w.o0), a.ref.r[Q]);
-    else v = Store<S>::load(*at<S>(rb, QB, w.om), a.ref.r[Q]);
-    // x boundaries: only the first and last tile of a row hold such sites (wave-uniform tests)
-    if constexpr (cx == 1) {
-        if (w.ti == 0 && w.i == 0) {
-            if constexpr (XOPEN) v = a.inl[Q];
-            else v = Store<S>::load(*at<S>(rb, QB + (int64_t)(a.tpr - 1) * TB, (uint32_t)((a.l - 1) & 63) * (uint32_t)sizeof(S)), a.ref.r[Q]);
-        }
-    }
-    if constexpr (cx == -1) {
-        if (w.ti == a.tpr - 1 && w.i == a.l - 1) {
-            if constexpr (XOPEN) v = a.out_cur[(Q - 9) * (size_t)(a.m * a.n) + w.jj[1 - cy] + (size_t)a.m * w.kk[1 - cz]];
-            else v = Store<S>::load(*at<S>(rb, QB - (int64_t)w.ti * TB, 0u), a.ref.r[Q]);
-        }
-    }
+    const S *p;
+    if constexpr (cx == 1) p = at<S>(rb, QB - TB, w.op);
+    else if constexpr (cx == 0) p = at<S>(rb, QB, w.o0);
+    else p = at<S>(rb, QB, w.om);
+    float ref = a.ref.r[Q];
     if constexpr (MIXED) {
         // the source is solid: bounce back the site's own opposite population
-        if ((lm >> Q) & 1u) v = Store<S>::load(*at<S>(w.row[1][1], (int64_t)op * kTile * sizeof(S), w.o0), a.ref.r[op]);
+        if ((lm >> Q) & 1u) {
+            p = at<S>(w.row[1][1], (int64_t)op * kTile * sizeof(S), w.o0);
+            ref = a.ref.r[op];
+        }
     }
-    x[Q] = v;
+    x[Q] = Store<S>::load(*p, ref);
 }
 
-template <typename S, bool XOPEN, bool MIXED, int... Qs>
+template <typename S, bool MIXED, int... Qs>
 __device__ __forceinline__ void pull_all(QSeq<Qs...>, const XArgs<S> &a, const Wave &w, uint32_t lm,
                                          typename Store<S>::compute *x)
 {
-    (pull<S, XOPEN, MIXED, Qs>(a, w, lm, x), ...);
+    (pull<S, MIXED, Qs>(a, w, lm, x), ...);
+}
+
+// Second pass, populations with c_x = +1 (Q = 0..4) at the site x = 0 and c_x = -1 (Q = 9..13) at
+// x = l - 1: open ends take the inlet equilibrium and the outlet ghost plane, a periodic x takes
+// the far end of the row (unless the bounce-back of the first pass applies).  Only the first and
+// the last tile of a row reach this code (wave-uniform tests).
+template <typename S, bool XOPEN>
+__device__ __forceinline__ void x_edges(const XArgs<S> &a, const Wave &w, uint32_t lm, typename Store<S>::compute *x)
+{
+    using R = typename Store<S>::compute;
+    constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
+    constexpr int cxs[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+    constexpr int cys[19] = {-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1, 0};
+    constexpr int czs[19] = {0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1, 0};
+    if (w.ti == 0) {
+        const bool edge = w.i == 0;
+#pragma unroll
+        for (int Q = 0; Q < 5; ++Q) {
+            if constexpr (XOPEN) {
+                x[Q] = edge ? a.inl[Q] : x[Q];
+            } else {
+                if (edge && !((lm >> Q) & 1u)) {
+                    const char *rb = w.row[1 - cys[Q]][1 - czs[Q]];
+                    x[Q] = Store<S>::load(*at<S>(rb, (int64_t)Q * kTile * sizeof(S) + (int64_t)(a.tpr - 1) * TB,
+                                                 (uint32_t)((a.l - 1) & 63) * (uint32_t)sizeof(S)),
+                                          a.ref.r[Q]);
+                }
+            }
+        }
+    }
+    if (w.ti == a.tpr - 1) {
+        const bool edge = w.i == a.l - 1;
+#pragma unroll
+        for (int Q = 9; Q < 14; ++Q) {
+            if constexpr (XOPEN) {
+                const R o = a.out_cur[(Q - 9) * (size_t)(a.m * a.n) + w.jj[1 - cys[Q]] +
+                                      (size_t)a.m * w.kk[1 - czs[Q]]];
+                x[Q] = edge ? o : x[Q];
+            } else {
+                if (edge && !((lm >> Q) & 1u)) {
+                    const char *rb = w.row[1 - cys[Q]][1 - czs[Q]];
+                    x[Q] = Store<S>::load(*at<S>(rb, (int64_t)Q * kTile * sizeof(S) - (int64_t)w.ti * TB, 0u),
+                                          a.ref.r[Q]);
+                }
+            }
+        }
+    }
+    (void)cxs;
 }
 
 #ifndef EUP_NT_STORE
@@ -160,10 +205,10 @@ __device__ __forceinline__ void put_all(QSeq<Qs...>, const XArgs<S> &a, char *tb
     (put<S, Qs>(a, tb, o0, x), ...);
 }
 
-// Update NT adjacent tiles t .. t+NT-1 of one row, one site per lane in each.  MIXED (NT = 1 only)
-// applies the link mask lm of the lane's site: bounce-back from solid sources, solid sites frozen.
-template <typename S, bool XOPEN, bool MIXED, int NT>
-__device__ __forceinline__ void update_tiles(const XArgs<S> &a, uint32_t t, uint32_t lm)
+// Update tile t, one site per lane.  MIXED applies the link mask lm of the lane's site: bounce-back
+// from solid sources, solid sites frozen.
+template <typename S, bool XOPEN, bool MIXED>
+__device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, uint32_t lm)
 {
     using R = typename Store<S>::compute;
     constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
@@ -174,84 +219,42 @@ __device__ __forceinline__ void update_tiles(const XArgs<S> &a, uint32_t t, uint
         if (lm & 0x80000000u) return; // solid site: frozen
     }
     const int j = (int)(row % (uint32_t)a.m), k = (int)(row / (uint32_t)a.m);
-    Wave w[NT];
-    R x[NT][19];
+    Wave w;
+    wave_setup<S>(a, w, ti, j, k, lane);
+    R x[19];
+    pull_all<S, MIXED>(Q18{}, a, w, lm, x);
+    x[18] = Store<S>::load(*at<S>(w.row[1][1], 18 * kTile * (int64_t)sizeof(S), w.o0), a.ref.r[18]);
+    x_edges<S, XOPEN>(a, w, lm, x);
+    if (w.i >= a.l) return; // padding lanes of a partial last tile
+    collide<R>(x, x, a.omega, a.fgrad);
+    put_all<S>(Q19{}, a, reinterpret_cast<char *>(a.B) + (int64_t)t * TB, w.o0, x);
+    if constexpr (XOPEN) {
+        if (w.i == a.l - 1) {
+            R o[5];
+            outlet_eq<R>(x, o);
+            const size_t mn = (size_t)a.m * a.n;
 #pragma unroll
-    for (int u = 0; u < NT; ++u) {
-        wave_setup<S>(a, w[u], ti + u, j, k, lane);
-        pull_all<S, XOPEN, MIXED>(Q18{}, a, w[u], lm, x[u]);
-        x[u][18] = Store<S>::load(*at<S>(w[u].row[1][1], 18 * kTile * (int64_t)sizeof(S), w[u].o0), a.ref.r[18]);
-    }
-    char *tb = reinterpret_cast<char *>(a.B) + (int64_t)t * TB;
-#pragma unroll
-    for (int u = 0; u < NT; ++u) {
-        if (w[u].i >= a.l) continue; // padding lanes of a partial last tile
-        collide<R>(x[u], x[u], a.omega, a.fgrad);
-        put_all<S>(Q19{}, a, tb + u * TB, w[u].o0, x[u]);
-        if constexpr (XOPEN) {
-            if (w[u].i == a.l - 1) {
-                R o[5];
-                outlet_eq<R>(x[u], o);
-                const size_t mn = (size_t)a.m * a.n;
-#pragma unroll
-                for (int v = 0; v < 5; ++v) a.out_next[v * mn + j + (size_t)a.m * k] = o[v];
-            }
+            for (int v = 0; v < 5; ++v) a.out_next[v * mn + j + (size_t)a.m * k] = o[v];
         }
-    }
-}
-
-// Tiles per wavefront: NT adjacent pure-fluid tiles of one row are updated together (NT
-// independent loads per population per lane in flight); other tiles one at a time.
-template <typename S> struct TilesPerWave {
-    static constexpr int value = sizeof(S) == 2 ? 2 : 1;
-};
-#ifdef EUP_TPW
-#define EUP_TPW_OF(S) EUP_TPW
-#else
-#define EUP_TPW_OF(S) TilesPerWave<S>::value
-#endif
-
-template <typename S, bool XOPEN>
-__device__ __forceinline__ void do_tile(const XArgs<S> &a, uint32_t t, int32_t code)
-{
-    if (code == kTileSolid) return;
-    if (code == kTileFluid) {
-        update_tiles<S, XOPEN, false, 1>(a, t, 0u);
-    } else {
-        const uint32_t lm = a.link[(size_t)code * kTile + (threadIdx.x & 63)];
-        update_tiles<S, XOPEN, true, 1>(a, t, lm);
     }
 }
 
 template <typename S, bool XOPEN>
 __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
 {
-    constexpr int kTilesPerWave = EUP_TPW_OF(S);
     // XCD-aware block order: hardware block b runs on XCD b % 8; give each XCD a contiguous
     // range of tiles so that the lines shared by neighbouring tiles stay in one L2.
     const uint32_t b = blockIdx.x;
     const uint32_t lb = (b & 7u) * a.chunk + (b >> 3);
-    const uint32_t w = __builtin_amdgcn_readfirstlane(lb * (kBlock / kTile) + (threadIdx.x >> 6));
-    const uint32_t t0 = w * kTilesPerWave;
-    if (t0 >= a.tiles) return;
-    const int32_t c0 = __builtin_amdgcn_readfirstlane(a.tile_code[t0]);
-    if constexpr (kTilesPerWave == 1) {
-        do_tile<S, XOPEN>(a, t0, c0);
+    const uint32_t t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kTile) + (threadIdx.x >> 6));
+    if (t >= a.tiles) return;
+    const int32_t code = __builtin_amdgcn_readfirstlane(a.tile_code[t]);
+    if (code == kTileSolid) return;
+    if (code == kTileFluid) {
+        update_tile<S, XOPEN, false>(a, t, 0u);
     } else {
-        const uint32_t seg = t0 % (uint32_t)a.tpr;
-        bool all_fluid = c0 == kTileFluid && seg + kTilesPerWave <= (uint32_t)a.tpr;
-#pragma unroll
-        for (int u = 1; u < kTilesPerWave; ++u)
-            all_fluid = all_fluid && t0 + u < a.tiles && a.tile_code[t0 + u] == kTileFluid;
-        if (__builtin_amdgcn_readfirstlane((int)all_fluid)) {
-            update_tiles<S, XOPEN, false, kTilesPerWave>(a, t0, 0u);
-        } else {
-            do_tile<S, XOPEN>(a, t0, c0);
-#pragma unroll
-            for (int u = 1; u < kTilesPerWave; ++u)
-                if (t0 + u < a.tiles)
-                    do_tile<S, XOPEN>(a, t0 + u, __builtin_amdgcn_readfirstlane(a.tile_code[t0 + u]));
-        }
+        const uint32_t lm = a.link[(size_t)code * kTile + (threadIdx.x & 63)];
+        update_tile<S, XOPEN, true>(a, t, lm);
     }
 }
 
@@ -429,7 +432,7 @@ template <typename S> XArgs<S> make_args(Handle *h)
     a.n = h->g.n;
     a.tpr = h->g.tiles_per_row;
     a.tiles = (uint32_t)h->g.tiles;
-    const uint32_t per_block = (kBlock / kTile) * EUP_TPW_OF(S);
+    const uint32_t per_block = kBlock / kTile;
     const uint32_t nb = (uint32_t)((h->g.tiles + per_block - 1) / per_block);
     a.chunk = (nb + 7) / 8;
     return a;

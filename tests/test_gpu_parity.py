@@ -112,6 +112,7 @@ def interior(p):
 
 @pytest.mark.parametrize("kind,x_open,shape", [("cylinder", True, (70, 24, 20)),
                                                 ("cylinder", False, (64, 24, 20)),
+                                                ("sponge", False, (100, 24, 20)),
                                                 ("sponge", True, (128, 40, 48))])
 def test_physics_f64_equals_oracle(kind, x_open, shape):
     l, m, n = shape
@@ -131,16 +132,18 @@ def test_physics_f64_equals_oracle(kind, x_open, shape):
     assert not bad.any(), f"{bad.sum()} values differ, max abs {np.abs(a - b).max()}"
 
 
+@pytest.mark.parametrize("shape,x_open", [((128, 48, 48), True), ((200, 24, 24), False)])
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
-def test_physics_tile_skip_is_exact(storage):
-    l, m, n = 128, 48, 48
-    announce(f"physics {storage} sponge {l}x{m}x{n}: tile skip vs no skip")
+def test_physics_tile_skip_is_exact(storage, shape, x_open):
+    """The pure-fluid tile path against the per-site mixed path (every tile marked mixed)."""
+    l, m, n = shape
+    announce(f"physics {storage} sponge {l}x{m}x{n} x_open={x_open}: tile skip vs no skip")
     mask, pops = physics_case("sponge", l, m, n, seed=3)
     svisc = (2 * 0.52 - 1) / 6
     out = []
     for skip in (True, False):
         with lattice((l, m, n), svisc=svisc, u_inflow=0.06, mode="physics", storage=storage,
-                     tile_skip=skip) as lat:
+                     tile_skip=skip, x_open=x_open) as lat:
             lat.set_mask(mask)
             lat.set_state(pops)
             lat.step(20)
