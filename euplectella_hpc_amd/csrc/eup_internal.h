@@ -57,6 +57,7 @@ struct Handle {
     int32_t *mixed_list = nullptr; // tile indices of mixed tiles
     void *outlet[2] = {nullptr, nullptr}; // outlet ghost plane, 5 x m x n (compute type)
     bool outlet_dirty = true;
+    size_t fused_lds = (size_t)-1; // dynamic LDS per block of the fused kernel (occupancy cap)
     // diagnostics scratch
     double *diag = nullptr;       // device scratch for reductions
     int64_t diag_len = 0;
