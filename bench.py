@@ -11,12 +11,19 @@ the algorithmic HBM bytes of the fluid sites actually updated (19 loads + 19 sto
 type each; solid sites are neither loaded nor stored), divided by the kernel time measured with HIP
 events on the library's stream.
 
+Beside the nominal 8 TB/s peak the line carries the HBM rates the same GPU reaches in plain
+streams (eup_membw: copy / read / write of 2 GiB, one 16-byte vector per lane), and
+``roofline.traffic``, the HBM bytes per launch of the fused kernel that rocprofv3's FETCH_SIZE and
+WRITE_SIZE counters gave for this workload and storage (calibrated on the copy stream; the summary
+is read from profiles/, null when no profile of this configuration is committed).
+
 CPU baseline: the reference's own OpenMP executable (built from its sources by
 oracle/Makefile.ref) on a 512 x 256 x 64 slab of the same sponge, 30 steps, its own MLUPS report.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -124,6 +131,27 @@ def run_one(wl, storage, mask, steps, warmup, chain=True):
         }
     finally:
         lat.close()
+
+
+def hbm_streams(mib=2048, reps=10):
+    """GB/s of the plain copy / read / write streams on this GPU (eup_membw)."""
+    from euplectella_hpc_amd.lib import membw
+    r = membw(0, mib, reps)
+    return {k: round(float(v), 1) for k, v in r.items()}
+
+
+def profiled_traffic(workload, storage):
+    """HBM bytes per fused-kernel launch from the committed rocprofv3 counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"k_fused_{workload}_hbm_counters.txt")
+    if not os.path.exists(path):
+        return None, None
+    pat = re.compile(r"^traffic storage=(\S+) .*\bbytes=(\d+)")
+    with open(path) as f:
+        for line in f:
+            m = pat.match(line.strip())
+            if m and m.group(1) == storage:
+                return int(m.group(2)), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def _torch_sync():
@@ -245,6 +273,8 @@ def main(argv=None):
     ap.add_argument("--every", type=int, default=100, help="drag sampling interval (--accuracy)")
     ap.add_argument("--no-chain", action="store_true",
                     help="skip the higher-viscosity start-up chain (short profiling runs)")
+    ap.add_argument("--no-streams", action="store_true",
+                    help="skip the plain HBM copy / read / write streams")
     args = ap.parse_args(argv)
     if args.accuracy:
         r = accuracy(WORKLOADS[args.workload], ["f32", "f16", "f16s", "bf16", "bf16s"], args.accuracy,
@@ -277,6 +307,8 @@ def main(argv=None):
         sec = run_one(wl, args.secondary, mask, args.steps, args.warmup, not args.no_chain)
         print(f"[bench] {args.secondary}: {sec['mlups']:.0f} MLUPS, kernel {sec['kernel_ms']:.3f} ms",
               file=sys.stderr, flush=True)
+    streams = None if args.no_streams else hbm_streams()
+    traffic, traffic_src = profiled_traffic(args.workload, args.storage)
     cpu = None if args.no_cpu else cpu_baseline(wl)
     out = {
         "metric": "MLUPS + % HBM peak, FP32 vs FP16-storage at |dC_D|<=1e-3 vs FP64",
@@ -298,8 +330,13 @@ def main(argv=None):
         "mflups_fluid": round(prim["mflups_fluid"], 1),
         "roofline": {"bound": "hbm", "achieved": round(prim["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(prim["frac"], 4),
-                     "traffic": None, "kernel": "k_fused", "kernel_ms": round(prim["kernel_ms"], 4),
-                     "bytes_per_site": prim["bytes_per_site"]},
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes": int(prim["fluid_sites"] * prim["bytes_per_site"]),
+                     "kernel": "k_fused", "kernel_ms": round(prim["kernel_ms"], 4),
+                     "bytes_per_site": prim["bytes_per_site"],
+                     "hbm_streams_gbs": streams,
+                     "frac_of_copy": (round(prim["achieved_gbs"] / streams["copy"], 4)
+                                      if streams else None)},
         "tiles": prim["tiles"],
         "cpu_baseline": cpu,
     }

@@ -9,27 +9,28 @@ namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// one 16-byte vector per lane, one launch over the whole buffer (the fastest streaming form on
+// MI355X; grid-stride loops and several vectors per lane are slower)
 __global__ __launch_bounds__(256) void k_copy4(const v4f *__restrict__ a, v4f *__restrict__ b, int64_t n4)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
-        __builtin_nontemporal_store(a[i], &b[i]);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) __builtin_nontemporal_store(a[i], &b[i]);
 }
 
 __global__ __launch_bounds__(256) void k_read4(const v4f *__restrict__ a, int64_t n4, float *__restrict__ sink)
 {
-    float acc = 0.f;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x) {
-        const v4f v = a[i];
-        acc += v.x + v.y + v.z + v.w;
-    }
-    if (acc == 1234.5f) sink[blockIdx.x] = acc; // keeps the loads; never true for the fill value
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    const v4f v = a[i];
+    const float acc = v.x + v.y + v.z + v.w;
+    if (acc == 1234.5f) sink[blockIdx.x & 4095] = acc; // keeps the loads; never true for the fill value
 }
 
 __global__ __launch_bounds__(256) void k_write4(v4f *__restrict__ b, int64_t n4)
 {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const v4f v = {0.f, 0.f, 0.f, 0.f};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
-        __builtin_nontemporal_store(v, &b[i]);
+    if (i < n4) __builtin_nontemporal_store(v, &b[i]);
 }
 
 } // namespace
@@ -39,7 +40,8 @@ using namespace eup;
 
 extern "C" int eup_membw(int32_t device, int64_t bytes, int32_t reps, double out[3])
 {
-    if (!out || bytes < (1 << 20) || reps < 1) return fail(nullptr, EUP_EINVAL, "eup_membw: bad argument");
+    if (!out || bytes < (1 << 20) || bytes > ((int64_t)1 << 40) || reps < 1)
+        return fail(nullptr, EUP_EINVAL, "eup_membw: bad argument");
     if (hipSetDevice(device) != hipSuccess) return fail(nullptr, EUP_EHIP, "eup_membw: hipSetDevice");
     const int64_t n4 = bytes / 16;
     v4f *a = nullptr, *b = nullptr;
@@ -51,9 +53,7 @@ extern "C" int eup_membw(int32_t device, int64_t bytes, int32_t reps, double out
         hipMalloc((void **)&sink, 1 << 16) != hipSuccess || hipStreamCreate(&st) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = fail(nullptr, EUP_EHIP, "eup_membw: allocation failed");
-    int ncu = 256;
-    if (!rc) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
-    const unsigned grid = (unsigned)(ncu * 16);
+    const unsigned grid = (unsigned)((n4 + 255) / 256);
     for (int kind = 0; kind < 3 && !rc; ++kind) {
         auto launch = [&]() {
             if (kind == 0) hipLaunchKernelGGL(k_copy4, dim3(grid), dim3(256), 0, st, a, b, n4);
