@@ -93,8 +93,13 @@ def build_lattice(wl, storage, mask, chain=True):
     # u = (u_in, +-1e-4 seeded noise, 0) at every site, generated on the device
     lat.init_equilibrium(wl["u_in"], 0.0, 0.0, 1e-4, wl["seed"])
     for tau, steps in (wl.get("chain", ()) if chain else ()):
+        if tau <= wl["tau"]:
+            break  # the chain only runs stages of higher viscosity than the target
         lat.set_svisc(svisc_from_tau(tau))
         lat.step(steps)
+        lat.sync()
+        print(f"[bench] {storage}: start-up stage tau = {tau} ({steps} steps) done", file=sys.stderr,
+              flush=True)
     lat.set_svisc(svisc_from_tau(wl["tau"]))
     return lat
 
@@ -239,7 +244,8 @@ def accuracy(wl, storages, steps, every, window=0.5):
               file=sys.stderr, flush=True)
     cut = int(len(hist["f64"][0]) * (1 - window))
     ref_cd = hist["f64"][0][cut:].mean()
-    out = {"workload": wl_name(wl), "steps": steps, "every": every, "cd_f64": ref_cd,
+    out = {"workload": wl_name(wl), "tau": wl["tau"], "u_in": wl["u_in"], "steps": steps,
+           "every": every, "window": window, "cd_f64": ref_cd,
            "finite_f64": bool(np.all(np.isfinite(hist["f64"][0])))}
     for st, (cd, pr) in hist.items():
         if st == "f64":
@@ -275,7 +281,11 @@ def main(argv=None):
                     help="skip the higher-viscosity start-up chain (short profiling runs)")
     ap.add_argument("--no-streams", action="store_true",
                     help="skip the plain HBM copy / read / write streams")
+    ap.add_argument("--tau", type=float, default=0.0,
+                    help="override the workload's relaxation time (accuracy studies)")
     args = ap.parse_args(argv)
+    if args.tau:
+        WORKLOADS[args.workload] = dict(WORKLOADS[args.workload], tau=args.tau)
     if args.accuracy:
         r = accuracy(WORKLOADS[args.workload], ["f32", "f16", "f16s", "bf16", "bf16s"], args.accuracy,
                      args.every)
@@ -295,6 +305,8 @@ def main(argv=None):
 
     wl = WORKLOADS[args.workload]
     mask = workload_mask(wl)
+    print(f"[bench] {args.workload}: geometry ready, {int(mask.sum())} solid sites", file=sys.stderr,
+          flush=True)
     prim = run_one(wl, args.storage, mask, args.steps, args.warmup, not args.no_chain)
     if not prim["finite"]:
         print(f"bench.py: the {args.storage} state is not finite after the timed steps",
