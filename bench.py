@@ -257,6 +257,47 @@ def accuracy(wl, storages, steps, every, window=0.5):
     return out
 
 
+def stability(wl, steps, every, chain):
+    """First step (checked every ``every`` steps) at which an FP64 run of the workload stops being
+    finite at the probe site and in the momentum-exchange force, in parity mode (the reference's
+    serial step, bit for bit) and in physics mode; None = finite throughout."""
+    from euplectella_hpc_amd.lib import Lattice, svisc_from_tau
+
+    mask = workload_mask(wl)
+    probe = (wl["l"] * 8 // 20, wl["m"] * 9 // 16, wl["n"] // 2 - 5)
+    out = {"workload": wl_name(wl), "tau": wl["tau"], "u_in": wl["u_in"], "steps": steps,
+           "every": every, "start": "viscosity ramp" if chain else "impulsive (u = u_in everywhere)"}
+    for mode in ("parity", "physics"):
+        lat = Lattice(wl["l"], wl["m"], wl["n"], svisc=svisc_from_tau(wl["tau"]), u_inflow=wl["u_in"],
+                      mode=mode, storage="f64")
+        try:
+            lat.set_mask(mask)
+            lat.init_equilibrium(wl["u_in"], 0.0, 0.0, 1e-4, wl["seed"])
+            for tau, n in (wl.get("chain", ()) if chain else ()):
+                if tau <= wl["tau"]:
+                    break
+                lat.set_svisc(svisc_from_tau(tau))
+                lat.step(n)
+            lat.set_svisc(svisc_from_tau(wl["tau"]))
+            bad, done = None, 0
+            while done < steps:
+                k = min(every, steps - done)
+                lat.step(k)
+                done += k
+                ok = np.all(np.isfinite(lat.probe(*probe)))
+                if mode == "physics":
+                    ok = ok and np.all(np.isfinite(lat.drag_mem()))
+                if not ok:
+                    bad = done
+                    break
+        finally:
+            lat.close()
+        print(f"[stability] {mode} f64: " + (f"not finite by step {bad}" if bad else f"finite for {steps} steps"),
+              file=sys.stderr, flush=True)
+        out[f"{mode}_f64_first_nonfinite_step"] = bad
+    return out
+
+
 def wl_name(wl):
     return next(k for k, v in WORKLOADS.items() if v is wl)
 
@@ -281,11 +322,17 @@ def main(argv=None):
                     help="skip the higher-viscosity start-up chain (short profiling runs)")
     ap.add_argument("--no-streams", action="store_true",
                     help="skip the plain HBM copy / read / write streams")
+    ap.add_argument("--stability", type=int, default=0, metavar="STEPS",
+                    help="only report when FP64 runs (parity and physics mode) stop being finite")
     ap.add_argument("--tau", type=float, default=0.0,
                     help="override the workload's relaxation time (accuracy studies)")
     args = ap.parse_args(argv)
     if args.tau:
         WORKLOADS[args.workload] = dict(WORKLOADS[args.workload], tau=args.tau)
+    if args.stability:
+        print(json.dumps(stability(WORKLOADS[args.workload], args.stability, args.every,
+                                   not args.no_chain)))
+        return 0
     if args.accuracy:
         r = accuracy(WORKLOADS[args.workload], ["f32", "f16", "f16s", "bf16", "bf16s"], args.accuracy,
                      args.every)

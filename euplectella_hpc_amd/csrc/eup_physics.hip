@@ -207,22 +207,17 @@ __device__ __forceinline__ void put_all(QSeq<Qs...>, const XArgs<S> &a, char *tb
     (put<S, Qs>(a, tb, o0, x), ...);
 }
 
-// Update tile t, one site per lane.  MIXED applies the link mask lm of the lane's site: bounce-back
-// from solid sources, solid sites frozen.
+// Update tile t (addressing w, row j, k), one site per lane.  MIXED applies the link mask lm of the
+// lane's site: bounce-back from solid sources, solid sites frozen.
 template <typename S, bool XOPEN, bool MIXED>
-__device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, uint32_t lm)
+__device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, const Wave &w, int j, int k,
+                                            uint32_t lm)
 {
     using R = typename Store<S>::compute;
     constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
-    const uint32_t row = t / (uint32_t)a.tpr;
-    const int ti = (int)(t - row * (uint32_t)a.tpr);
-    const int lane = (int)(threadIdx.x & 63);
     if constexpr (MIXED) {
         if (lm & 0x80000000u) return; // solid site: frozen
     }
-    const int j = (int)(row % (uint32_t)a.m), k = (int)(row / (uint32_t)a.m);
-    Wave w;
-    wave_setup<S>(a, w, ti, j, k, lane);
     R x[19];
     pull_all<S, MIXED>(Q18{}, a, w, lm, x);
     x[18] = Store<S>::load(*at<S>(w.row[1][1], 18 * kTile * (int64_t)sizeof(S), w.o0), a.ref.r[18]);
@@ -250,13 +245,22 @@ __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
     const uint32_t lb = (b & 7u) * a.chunk + (b >> 3);
     const uint32_t t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kTile) + (threadIdx.x >> 6));
     if (t >= a.tiles) return;
-    const int32_t code = __builtin_amdgcn_readfirstlane(a.tile_code[t]);
+    // the codes are read-only during the launch: a scalar load through the constant address space
+    // (consecutive wavefronts share its scalar-cache line) instead of a vector load + readfirstlane;
+    // the addressing below does not depend on the code and overlaps the load
+    typedef const __attribute__((address_space(4))) int32_t ConstI32;
+    const int32_t code = *(ConstI32 *)(a.tile_code + t);
+    const uint32_t row = t / (uint32_t)a.tpr;
+    const int ti = (int)(t - row * (uint32_t)a.tpr);
+    const int j = (int)(row % (uint32_t)a.m), k = (int)(row / (uint32_t)a.m);
+    Wave w;
+    wave_setup<S>(a, w, ti, j, k, (int)(threadIdx.x & 63));
     if (code == kTileSolid) return;
     if (code == kTileFluid) {
-        update_tile<S, XOPEN, false>(a, t, 0u);
+        update_tile<S, XOPEN, false>(a, t, w, j, k, 0u);
     } else {
         const uint32_t lm = a.link[(size_t)code * kTile + (threadIdx.x & 63)];
-        update_tile<S, XOPEN, true>(a, t, lm);
+        update_tile<S, XOPEN, true>(a, t, w, j, k, lm);
     }
 }
 
