@@ -53,6 +53,12 @@ WORKLOADS = {
                                 ridge_width=9, diameter=315.0,
                                 chain=((0.6, 1000), (0.55, 500), (0.52, 500), (0.505, 500))),
 }
+# one GPU's share of the capacity run in an 8-way x-slab decomposition: the 192-wide slab
+# x = 288..479 of its geometry (it holds the sponge, centred at x = 384), with periodic x faces
+# standing in for the neighbours' halo planes
+WORKLOADS["sponge_1536x768x768_slab192"] = dict(WORKLOADS["sponge_1536x768x768"], l=192,
+                                                parent="sponge_1536x768x768", crop_x=(288, 480),
+                                                x_open=False)
 
 # Start-up: plain BGK at tau close to 1/2 does not survive the impulsive start u = u_in everywhere
 # (the FP64 run of the 512 x 256 x 256 sponge at tau = 0.505 diverges after about 170 steps).
@@ -63,6 +69,9 @@ WORKLOADS = {
 
 def workload_mask(wl, nz=None):
     from euplectella_hpc_amd import geometry
+    if "crop_x" in wl:
+        a, b = wl["crop_x"]
+        return np.ascontiguousarray(workload_mask(WORKLOADS[wl["parent"]], nz)[:, :, a:b])
     if wl.get("kind") == "cylinder":
         return geometry.cylinder(wl["l"], wl["m"], nz or wl["n"], wl["radius"], wl["centre"])
     if wl.get("kind") == "ridged":
@@ -88,7 +97,7 @@ def build_lattice(wl, storage, mask, chain=True):
     l, m, n = wl["l"], wl["m"], wl["n"]
     st, shifted = parse_storage(storage)
     lat = Lattice(l, m, n, svisc=svisc_from_tau(wl["tau"]), u_inflow=wl["u_in"], mode="physics",
-                  storage=st, shifted=shifted)
+                  storage=st, shifted=shifted, x_open=wl.get("x_open", True))
     lat.set_mask(mask)
     # u = (u_in, +-1e-4 seeded noise, 0) at every site, generated on the device
     lat.init_equilibrium(wl["u_in"], 0.0, 0.0, 1e-4, wl["seed"])
@@ -385,7 +394,9 @@ def main(argv=None):
         "data": "synthetic (seeded sponge geometry, u = inlet + seeded noise)",
         "config": {"workload": args.workload, "lattice": [wl["l"], wl["m"], wl["n"]],
                    "tau": wl["tau"], "u_in": wl["u_in"], "storage": args.storage,
-                   "mode": "physics", "parallelism": "single GPU"},
+                   "mode": "physics", "parallelism": "single GPU",
+                   "x_faces": "open (inlet / outlet)" if wl.get("x_open", True) else
+                   f"periodic (stand-in for the halo of x-slab {wl['crop_x']} of {wl.get('parent')})"},
         "mflups_fluid": round(prim["mflups_fluid"], 1),
         "roofline": {"bound": "hbm", "achieved": round(prim["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(prim["frac"], 4),

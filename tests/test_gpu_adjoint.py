@@ -3,7 +3,7 @@
 * grey FP64 step == oracle grey step bit for bit; a binary field == physics mode on fluid sites;
 * objective and gradient (FP64) match the oracle's adjoint;
 * the FP32 adjoint matches the FP64 adjoint, and the FP64 adjoint matches central finite
-  differences of the FP64 objective (relative 1e-4 is the stated tolerance);
+  differences of the FP64 objective on 64 cells (relative 1e-4 is the stated tolerance);
 * the torch.autograd.Function returns the same gradient.
 """
 import numpy as np
@@ -113,7 +113,8 @@ def c4_lattice(storage):
 
 
 def test_c4_fp32_adjoint_matches_fp64_adjoint_and_finite_differences():
-    """C4: 128x64x64, 12 blurred spheres, mean C_D over steps 1000-2000, checkpoint every 100."""
+    """C4: 128x64x64, 12 blurred spheres, mean C_D over steps 1000-2000, checkpoint every 100;
+    central differences on 64 random cells of the grey transition zone."""
     c = C4
     announce("C4 grey 128x64x64: FP64 and FP32 objective + adjoint, 2000 steps")
     s, pops, norm = c4_inputs()
@@ -131,8 +132,8 @@ def test_c4_fp32_adjoint_matches_fp64_adjoint_and_finite_differences():
         eps = 1e-5
         cand = np.flatnonzero((np.abs(g.ravel()) > 0.01 * np.abs(g).max()) & (sf > 4 * eps) &
                               (sf < 1 - 4 * eps))
-        assert cand.size >= 12
-        cells = rng.choice(cand, 12, replace=False)
+        assert cand.size >= 64
+        cells = rng.choice(cand, 64, replace=False)
         worst = 0.0
         for cidx in cells:
             Js = []

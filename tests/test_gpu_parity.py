@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from golden_cases import CASES, make_inputs, omega_of, sha, e14_6
+from momentum_balance import box_momentum
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -198,6 +199,37 @@ def test_physics_drag_mem_and_diagno_match_oracle():
     assert np.allclose(d, dr, rtol=1e-13, atol=1e-16), (d, dr)
     pr = orc.probe(ref, 40, 20, 5)
     assert np.array_equal(p, pr)
+
+
+def test_momentum_exchange_closes_the_momentum_balance():
+    """Discrete momentum balance of one step around the body: the momentum stored in a box after
+    the step minus before = inflow through its surface - the momentum-exchange force (collision
+    conserves momentum, bounce-back hands 2 c f to the body).  Checked on two boxes after a
+    developed start; the mirror-symmetric cylinder (no noise) has no lift.  R = 5, u_in = 0.02,
+    tau = 0.8 (Re = 2)."""
+    from euplectella_hpc_amd import geometry
+    from euplectella_hpc_amd.lib import svisc_from_tau
+    l, m, n = 96, 48, 4
+    announce(f"physics f64 cylinder {l}x{m}x{n}: momentum exchange closes the box balance")
+    mask = geometry.cylinder(l, m, n, 5, (24, (m + 1) / 2))
+    pops = geometry.initial_state(l, m, n, 0.02, noise=0.0)
+    with lattice((l, m, n), svisc=svisc_from_tau(0.8), u_inflow=0.02, mode="physics",
+                 storage="f64") as lat:
+        lat.set_mask(mask)
+        lat.set_state(pops)
+        lat.step(3000)
+        F = lat.drag_mem()
+        P0 = lat.get_state()
+        lat.step(1)
+        P1 = lat.get_state()
+    for box in ((8, 60, 8, 41), (14, 34, 14, 35)):
+        M0, G = box_momentum(P0, box)
+        M1, _ = box_momentum(P1, box)
+        err = (M1 - M0) - (G - F)
+        announce(f"  box {box}: F_mem_x {F[0]:.12e}, inflow_x {G[0]:.12e}, dM_x {M1[0] - M0[0]:.3e}, "
+                 f"balance error {np.abs(err).max():.2e}")
+        assert np.abs(err).max() < 1e-10 * abs(F[0])
+    assert abs(F[1]) < 1e-12 * abs(F[0]) and abs(F[2]) < 1e-12 * abs(F[0])
 
 
 def test_parity_diagno_and_mem_match_oracle(golden):

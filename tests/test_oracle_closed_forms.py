@@ -263,3 +263,23 @@ def test_flux_x_uniform_equilibrium_gives_pressure_only():
     # only face 2 inside (case B) -> norm(2) = 0 : fluxX = -F(i1) = +(#sites)*rho/3
     boxb = (3, l + 5, 2, 6, 1, n)
     assert abs(orc.flux_x(p, 1.5, boxb) - 5 * n * rho / 3) < 1e-13
+
+
+def test_mem_force_closes_the_momentum_balance():
+    """orc_mem_force against the one-step momentum balance of orc_physics_steps around a
+    mirror-symmetric cylinder (Re = 2): the balance closes to rounding and there is no lift."""
+    from momentum_balance import box_momentum
+    from euplectella_hpc_amd import geometry
+    l, m, n = 96, 48, 4
+    mask = geometry.cylinder(l, m, n, 5, (24, (m + 1) / 2))
+    P = geometry.initial_state(l, m, n, 0.02, noise=0.0)
+    om = orc.omega_from_svisc((2 * 0.8 - 1) / 6)
+    orc.physics_steps(P, mask, om, 0.02, 300)
+    P0 = P.copy()
+    F = orc.mem_force(P0, mask)
+    orc.physics_steps(P, mask, om, 0.02, 1)
+    for box in ((8, 60, 8, 41), (14, 34, 14, 35)):
+        M0, G = box_momentum(P0, box)
+        M1, _ = box_momentum(P, box)
+        assert np.abs((M1 - M0) - (G - F)).max() < 1e-10 * abs(F[0])
+    assert F[0] > 0 and abs(F[1]) < 1e-12 * F[0] and abs(F[2]) < 1e-12 * F[0]
