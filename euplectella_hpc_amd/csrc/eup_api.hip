@@ -420,6 +420,28 @@ int eup_probe(eup_handle *h, int32_t i, int32_t j, int32_t k, double out[4])
     return 0;
 }
 
+int eup_vtk_velocity(eup_handle *h, int32_t plane, int32_t index, float *out)
+{
+    int rc = ready(h, "eup_vtk_velocity");
+    if (rc) return rc;
+    if (!out) return fail(h, EUP_EINVAL, "eup_vtk_velocity: null argument");
+    if (plane < EUP_VTK_YZ || plane > EUP_VTK_3D) return fail(h, EUP_EINVAL, "eup_vtk_velocity: unknown plane");
+    const int extent = plane == EUP_VTK_YZ ? h->g.l : (plane == EUP_VTK_XZ ? h->g.m : h->g.n);
+    if (plane != EUP_VTK_3D && (index < 1 || index > extent))
+        return fail(h, EUP_ERANGE, "eup_vtk_velocity: plane index outside the lattice");
+    return diag_vtk_velocity(h, plane, index, out);
+}
+
+int eup_vtk_vorticity(eup_handle *h, int32_t k0, float *out)
+{
+    int rc = ready(h, "eup_vtk_vorticity");
+    if (rc) return rc;
+    if (!out) return fail(h, EUP_EINVAL, "eup_vtk_vorticity: null argument");
+    if (h->g.l < 3 || h->g.m < 3) return fail(h, EUP_EINVAL, "eup_vtk_vorticity: needs l, m >= 3");
+    if (k0 < 1 || k0 > h->g.n) return fail(h, EUP_ERANGE, "eup_vtk_vorticity: plane index outside the lattice");
+    return diag_vtk_vorticity(h, k0, out);
+}
+
 int eup_diagno(eup_handle *h, double out[5])
 {
     int rc = ready(h, "eup_diagno");

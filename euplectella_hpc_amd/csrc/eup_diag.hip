@@ -1,5 +1,6 @@
 // Diagnostics in FP64: probe (probe_global.F90:41-77), control-volume stress drag (flux_X.F90,
-// flux_Y.F90), momentum-exchange drag (no reference counterpart) and global means (diagno.F90).
+// flux_Y.F90), momentum-exchange drag (no reference counterpart), global means (diagno.F90) and the
+// velocity / vorticity fields of the binary VTK writers (vtk_*.binary.F90).
 //
 // Populations are converted to double on load whatever the storage.  The control-volume drag
 // computes one stress per face site on the GPU with the reference's expression order and returns
@@ -215,6 +216,73 @@ __global__ __launch_bounds__(kBlock) void k_mem_tiles(PhysAcc<S> acc, const int3
     block_reduce<3>(v, partial);
 }
 
+// velocity of one site as the reference's binary VTK writers compute it (vtk_xy.binary.F90:93-115):
+// momenta and density summed in FP64 in the written order, each rounded to float, divided in
+// float (IEEE division; the translation unit is built without contraction)
+template <typename Acc>
+__device__ __forceinline__ void vtk_site(const Acc &acc, int i, int j, int k, float *o)
+{
+    double x[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) x[q] = acc(q, i, j, k);
+    const float u = (float)(x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]);
+    const float w = (float)(x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]);
+    const float v = (float)(x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]);
+    const float den = (float)(x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9]
+                              + x[10] + x[11] + x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18]);
+    o[0] = u / den;
+    o[1] = w / den;
+    o[2] = v / den;
+}
+
+// one plane of velocities in the writers' loop order: plane 0 = yz at x = index (k, j), 1 = xz at
+// y = index (k, i), 2 = xy at z = index (j, i)
+template <typename Acc>
+__global__ void k_vtk_velocity(Acc acc, int plane, int index, float *out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = acc.l, m = acc.m, n = acc.n;
+    int i, j, k;
+    if (plane == 0) {
+        if (t >= (int64_t)m * n) return;
+        i = index; j = 1 + (int)(t % m); k = 1 + (int)(t / m);
+    } else if (plane == 1) {
+        if (t >= (int64_t)l * n) return;
+        i = 1 + (int)(t % l); j = index; k = 1 + (int)(t / l);
+    } else {
+        if (t >= (int64_t)l * m) return;
+        i = 1 + (int)(t % l); j = 1 + (int)(t / l); k = index;
+    }
+    vtk_site(acc, i, j, k, out + 3 * t);
+}
+
+// vorticity of the xy plane z = k0 as vtk_om.binary.F90:88-131 computes it (i = 2..l-1 fastest,
+// j = 2..m-1): densities rounded to float; in each momentum only the last population is divided
+// by the density, as the writer's expression reads; the difference is formed in float
+template <typename Acc>
+__global__ void k_vtk_vorticity(Acc acc, int k0, float *out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = acc.l, m = acc.m;
+    if (t >= (int64_t)(l - 2) * (m - 2)) return;
+    const int i = 2 + (int)(t % (l - 2)), j = 2 + (int)(t / (l - 2));
+    const int di[4] = {0, 0, -1, 1}, dj[4] = {1, -1, 0, 0}; // up, down, left, right
+    float mom[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        double x[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) x[q] = acc(q, i + di[s], j + dj[s], k0);
+        const float d = (float)(x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9]
+                                + x[10] + x[11] + x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18]);
+        if (s < 2)
+            mom[s] = (float)(x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13] / (double)d);
+        else
+            mom[s] = (float)(x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17] / (double)d);
+    }
+    out[t] = 0.5f * (mom[0] - mom[1]) - 0.5f * (mom[3] - mom[2]);
+}
+
 template <typename Acc>
 __global__ __launch_bounds__(kBlock) void k_diagno(Acc acc, const uint8_t *solid, double *partial)
 {
@@ -417,6 +485,58 @@ int diag_drag_mem(Handle *h, double F[3])
     hipLaunchKernelGGL((k_final<3>), dim3(1), dim3(kBlock), 0, h->stream, h->diag, kRedBlocks, res);
     if (hip_check(h, hipGetLastError(), "mem drag") ||
         hip_check(h, hipMemcpyAsync(F, res, sizeof(double) * 3, hipMemcpyDeviceToHost, h->stream), "copy") ||
+        hip_check(h, hipStreamSynchronize(h->stream), "sync"))
+        return EUP_EHIP;
+    return 0;
+}
+
+// one velocity plane into out (host), through a plane-sized device buffer
+static int vtk_plane(Handle *h, int plane, int index, float *out)
+{
+    const Geometry &g = h->g;
+    const int64_t count = plane == 0 ? (int64_t)g.m * g.n : (plane == 1 ? (int64_t)g.l * g.n : (int64_t)g.l * g.m);
+    int rc = ensure_diag(h, (3 * count + 1) / 2 + 1);
+    if (rc) return rc;
+    float *dev = reinterpret_cast<float *>(h->diag);
+    rc = with_acc(h, [&](auto acc) {
+        hipLaunchKernelGGL((k_vtk_velocity<decltype(acc)>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                           h->stream, acc, plane, index, dev);
+        return hip_check(h, hipGetLastError(), "vtk velocity") ? EUP_EHIP : 0;
+    });
+    if (rc) return rc;
+    if (hip_check(h, hipMemcpyAsync(out, dev, sizeof(float) * 3 * count, hipMemcpyDeviceToHost, h->stream),
+                  "copy vtk") ||
+        hip_check(h, hipStreamSynchronize(h->stream), "sync"))
+        return EUP_EHIP;
+    return 0;
+}
+
+int diag_vtk_velocity(Handle *h, int plane, int index, float *out)
+{
+    if (plane != 3) return vtk_plane(h, plane, index, out);
+    // the volume in the writer's order (k, j, i) = the xy planes one after the other
+    const int64_t per = (int64_t)h->g.l * h->g.m * 3;
+    for (int k = 1; k <= h->g.n; ++k) {
+        int rc = vtk_plane(h, 2, k, out + (k - 1) * per);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int diag_vtk_vorticity(Handle *h, int k0, float *out)
+{
+    const int64_t count = (int64_t)(h->g.l - 2) * (h->g.m - 2);
+    int rc = ensure_diag(h, count / 2 + 1);
+    if (rc) return rc;
+    float *dev = reinterpret_cast<float *>(h->diag);
+    rc = with_acc(h, [&](auto acc) {
+        hipLaunchKernelGGL((k_vtk_vorticity<decltype(acc)>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                           h->stream, acc, k0, dev);
+        return hip_check(h, hipGetLastError(), "vtk vorticity") ? EUP_EHIP : 0;
+    });
+    if (rc) return rc;
+    if (hip_check(h, hipMemcpyAsync(out, dev, sizeof(float) * count, hipMemcpyDeviceToHost, h->stream),
+                  "copy vtk") ||
         hip_check(h, hipStreamSynchronize(h->stream), "sync"))
         return EUP_EHIP;
     return 0;

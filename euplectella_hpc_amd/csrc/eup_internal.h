@@ -97,6 +97,8 @@ int diag_site(Handle *h, int i, int j, int k, double out[19]);
 int diag_drag_cv(Handle *h, const int32_t box[6], double *fx, double *fy);
 int diag_drag_mem(Handle *h, double F[3]);
 int diag_diagno(Handle *h, double out[5]);
+int diag_vtk_velocity(Handle *h, int plane, int index, float *out);
+int diag_vtk_vorticity(Handle *h, int k0, float *out);
 
 // Element index of population q of site (i, j, k) (0-based) in a dense state buffer.
 //  * SoA (grey mode): q * stride + i + l * (j + m * k).

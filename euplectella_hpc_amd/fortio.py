@@ -9,6 +9,8 @@ little-endian length marker before and after the payload.
 * ``restore.XX.YY.ZZ.bin`` / ``save.XX.YY.ZZ.TTTTTTTT.bin``: record ``itime`` (int32), then 19
   records, population q = 1..19, each ``(l+2)(m+2)(n+2)`` reals with i fastest
   (CODE/SRC/restore_raw.F90:43-63, CODE/SRC/save_raw.F90:54-74)
+* ``tec_{xy,xz,yz,3d,om}.RRRR.TTTTTTTT.vtk``: binary VTK files, ASCII header then big-endian
+  float32 data (CODE/SRC/vtk_*.binary.F90)
 """
 import os
 import re
@@ -181,3 +183,63 @@ def read_deck(path):
 def ensure_dir(path):
     os.makedirs(path, exist_ok=True)
     return path
+
+
+# ----------------------------------------------------------------------------------------------
+# binary VTK files (CODE/SRC/vtk_*.binary.F90): an ASCII header written with the writers' fixed
+# edit descriptors, then big-endian float32 data (access='STREAM', CONVERT="BIG_ENDIAN")
+# ----------------------------------------------------------------------------------------------
+
+VTK_YZ, VTK_XZ, VTK_XY, VTK_3D = 0, 1, 2, 3
+_VTK_TAG = {VTK_YZ: "yz", VTK_XZ: "xz", VTK_XY: "xy", VTK_3D: "3d"}
+
+
+def vtk_file_name(kind, itime, rank=0):
+    """'tec_<kind>.RRRR.TTTTTTTT.vtk' (vtk_xy.binary.F90:43-47); kind in xy, xz, yz, 3d, om."""
+    return f"tec_{kind}.{rank:04d}.{itime:08d}.vtk"
+
+
+def _vtk_header(dims, origin, count, data_lines):
+    # A26, A5, A6, A25; (A11,I10,A1,I10,A1,I10); (A7,...); (A8,...); (A10,I10) -- the literal
+    # 'POINT_DATA ' is cut to its first 10 characters by A10 -- and the data declaration in an Aw
+    # one wider than the text, so right-justified behind one blank (vtk_xy.binary.F90:51-64)
+    three = lambda v: "%10d %10d %10d" % tuple(v)
+    lines = ["# vtk DataFile Version 2.0", "Campo", "BINARY", "DATASET STRUCTURED_POINTS",
+             "DIMENSIONS " + three(dims), "ORIGIN " + three(origin), "SPACING " + three((1, 1, 1)),
+             "POINT_DATA" + "%10d" % count] + data_lines
+    return ("\n".join(lines) + "\n").encode("ascii")
+
+
+def vtk_velocity_bytes(plane, index, shape, data, offset=(0, 0, 0)):
+    """Bytes of the reference's velocity VTK file for ``plane`` (VTK_YZ / VTK_XZ / VTK_XY at the
+    1-based ``index``, or VTK_3D), ``shape`` = (l, m, n), ``data`` float32 [sites, 3] in the
+    writer's loop order (as eup_vtk_velocity returns it)."""
+    l, m, n = shape
+    ox, oy, oz = offset
+    dims, origin = {
+        VTK_YZ: ((1, m, n), (ox + 1 + index, oy + 1, oz + 1)),   # vtk_yz.binary.F90:55-59
+        VTK_XZ: ((l, 1, n), (ox + 1, oy + index + 1, oz + 1)),   # vtk_xz.binary.F90:55-59
+        VTK_XY: ((l, m, 1), (ox + 1, oy + 1, oz + 1 + index)),   # vtk_xy.binary.F90:55-59
+        VTK_3D: ((l, m, n), (ox + 1, oy + 1, oz + 1)),           # vtk_3d.binary.F90:58-62
+    }[plane]
+    count = dims[0] * dims[1] * dims[2]
+    arr = np.ascontiguousarray(data, dtype=np.float32).reshape(count, 3)
+    return _vtk_header(dims, origin, count, [" VECTORS velocity float"]) + arr.astype(">f4").tobytes()
+
+
+def vtk_vorticity_bytes(shape, data, offset=(0, 0, 0)):
+    """Bytes of the reference's vorticity VTK file (vtk_om.binary.F90:73-86): (l-2) x (m-2)
+    points with origin (2, 2, 1) whatever the plane, float32 data as eup_vtk_vorticity
+    returns it."""
+    l, m, _ = shape
+    ox, oy, oz = offset
+    count = (l - 2) * (m - 2)
+    arr = np.ascontiguousarray(data, dtype=np.float32).reshape(count)
+    return (_vtk_header((l - 2, m - 2, 1), (ox + 2, oy + 2, oz + 1), count,
+                        [" SCALARS vorticity float", "LOOKUP_TABLE default"])
+            + arr.astype(">f4").tobytes())
+
+
+def write_bytes(path, data):
+    with open(path, "wb") as fh:
+        fh.write(data)

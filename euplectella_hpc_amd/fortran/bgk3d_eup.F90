@@ -10,6 +10,10 @@
 !   probe_g.0000.dat        velocity probe every icheck steps (probe_global.F90:77,93;
 !                           site from diagnostic.F90:145)
 !   diagno.dat              global means at itime 0 for a fresh start (diagno.F90:157)
+!   tec_*.0000.T.vtk        binary VTK files every ivtim steps (diagnostic.F90:75-90): the default
+!                           branch (xy plane n/2-1, xz plane m/2-1, yz plane l/2-1, vorticity of
+!                           the xy plane n/2+1), or with EUP_VTK=hpc the -DHPC branch (3-D volume,
+!                           xy plane n/2+1); EUP_VTK=none writes none
 !   bgk.perf                MLUPS line (finalize.F90:107-108,222)
 ! Case: CYLINDER_INF + OBSTACLES (inflow at x = 0, outflow at x = l+1, periodic y and z;
 ! build.bcond.F90:319-339).  The step semantics and storage come from the environment:
@@ -35,7 +39,7 @@ program bgk3d_eup
    integer(c_int32_t) :: box(6)
    integer :: l, m, n, q, itime, itstart, nobs, idx, i, j, k, nxt, delta
    integer(8) :: c0, c1, crate
-   integer :: do_drag
+   integer :: do_drag, vtk_branch
    integer(c_int) :: rc
    character(len=32) :: env
    character(len=27) :: fname
@@ -73,6 +77,10 @@ program bgk3d_eup
    do_drag = 0
    call get_environment_variable('EUP_DRAG', env)
    if (trim(env) == '1') do_drag = 1
+   vtk_branch = 1
+   call get_environment_variable('EUP_VTK', env)
+   if (trim(env) == 'hpc') vtk_branch = 2
+   if (trim(env) == 'none') vtk_branch = 0
    cfg%front = EUP_FACE_OUTFLOW
    cfg%rear = EUP_FACE_INFLOW
    cfg%up = EUP_FACE_PERIODIC; cfg%down = EUP_FACE_PERIODIC
@@ -137,10 +145,22 @@ program bgk3d_eup
    call system_clock(c0, crate)
    itime = itstart
    do while (itime < itfin)
-      nxt = min(itfin, (itime / icheck + 1) * icheck, (itime / itsave + 1) * itsave)
+      nxt = min(itfin, (itime / icheck + 1) * icheck, (itime / itsave + 1) * itsave, &
+                (itime / ivtim + 1) * ivtim)
       rc = eup_step(h, nxt - itime)
       call eup_check(rc, h, 'eup_step')
       itime = nxt
+      if (mod(itime, ivtim) == 0) then
+         if (vtk_branch == 1) then
+            call vtk_velocity('xy', EUP_VTK_XY, n / 2 - 1)
+            call vtk_velocity('xz', EUP_VTK_XZ, m / 2 - 1)
+            call vtk_velocity('yz', EUP_VTK_YZ, l / 2 - 1)
+            call vtk_vorticity(n / 2 + 1)
+         else if (vtk_branch == 2) then
+            call vtk_velocity('3d', EUP_VTK_3D, 0)
+            call vtk_velocity('xy', EUP_VTK_XY, n / 2 + 1)
+         end if
+      end if
       if (mod(itime, icheck) == 0) then
          if (do_drag == 1) then
             do delta = 52, 67, 5
@@ -192,6 +212,73 @@ program bgk3d_eup
 1106 format(" # Mlups      ", 1(f14.6, 1x))
 
 contains
+
+   ! ASCII header of the binary VTK files, in the writers' edit descriptors (vtk_xy.binary.F90:51-64);
+   ! the file is then reopened as a big-endian stream for the data
+   subroutine vtk_header(fn, dims, origin, scalar)
+      character(len=*), intent(in) :: fn
+      integer, intent(in) :: dims(3), origin(3)
+      logical, intent(in) :: scalar
+      open(52, file=fn, status='unknown')
+      write(52, '(A26)') '# vtk DataFile Version 2.0'
+      write(52, '(A5)') 'Campo'
+      write(52, '(A6)') 'BINARY'
+      write(52, '(A25)') 'DATASET STRUCTURED_POINTS'
+      write(52, '(A11,I10,A1,I10,A1,I10)') 'DIMENSIONS ', dims(1), ' ', dims(2), ' ', dims(3)
+      write(52, '(A7,I10,A1,I10,A1,I10)') 'ORIGIN ', origin(1), ' ', origin(2), ' ', origin(3)
+      write(52, '(A8,I10,A1,I10,A1,I10)') 'SPACING ', 1, ' ', 1, ' ', 1
+      write(52, '(A10,I10)') 'POINT_DATA', dims(1) * dims(2) * dims(3)
+      if (scalar) then
+         write(52, '(A24)') 'SCALARS vorticity float'
+         write(52, '(A20)') 'LOOKUP_TABLE default'
+      else
+         write(52, '(A23)') 'VECTORS velocity float'
+      end if
+      close(52)
+      open(52, file=fn, status='old', position='append', form='unformatted', access='STREAM', &
+           convert='BIG_ENDIAN')
+   end subroutine vtk_header
+
+   ! velocity file of one plane (index = plane position) or of the volume (vtk_*_bin)
+   subroutine vtk_velocity(tag, plane, index)
+      character(len=2), intent(in) :: tag
+      integer(c_int32_t), intent(in) :: plane
+      integer, intent(in) :: index
+      character(len=24) :: fn
+      integer :: dims(3), origin(3)
+      real(c_float), allocatable :: vel(:)
+      select case (plane)
+      case (EUP_VTK_YZ); dims = [1, m, n]; origin = [1 + index, 1, 1]
+      case (EUP_VTK_XZ); dims = [l, 1, n]; origin = [1, 1 + index, 1]
+      case (EUP_VTK_XY); dims = [l, m, 1]; origin = [1, 1, 1 + index]
+      case default; dims = [l, m, n]; origin = [1, 1, 1]
+      end select
+      allocate(vel(3 * dims(1) * dims(2) * dims(3)))
+      rc = eup_vtk_velocity(h, plane, int(index, c_int32_t), vel)
+      call eup_check(rc, h, 'eup_vtk_velocity')
+      fn = 'tec_' // tag // '.0000.xxxxxxxx.vtk'
+      write(fn(13:20), '(i8.8)') itime
+      call vtk_header(fn, dims, origin, .false.)
+      write(52) vel
+      close(52)
+      deallocate(vel)
+   end subroutine vtk_velocity
+
+   ! vorticity file of the xy plane k0 (vtk_om_bin): (l-2) x (m-2) points, origin (2, 2, 1)
+   subroutine vtk_vorticity(k0)
+      integer, intent(in) :: k0
+      character(len=24) :: fn
+      real(c_float), allocatable :: vor(:)
+      allocate(vor((l - 2) * (m - 2)))
+      rc = eup_vtk_vorticity(h, int(k0, c_int32_t), vor)
+      call eup_check(rc, h, 'eup_vtk_vorticity')
+      fn = 'tec_om.0000.xxxxxxxx.vtk'
+      write(fn(13:20), '(i8.8)') itime
+      call vtk_header(fn, [l - 2, m - 2, 1], [2, 2, 1], .true.)
+      write(52) vor
+      close(52)
+      deallocate(vor)
+   end subroutine vtk_vorticity
 
    ! D3Q19 weight of population q (storage.F90:77-81; rest state of init.F90 opt 0)
    real(c_double) function weight(q)

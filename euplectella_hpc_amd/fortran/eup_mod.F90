@@ -14,6 +14,7 @@ module eup
    integer(c_int32_t), parameter :: EUP_FACE_PERIODIC = 0, EUP_FACE_INFLOW = 4
    integer(c_int32_t), parameter :: EUP_FACE_OUTFLOW = 5
    integer(c_int32_t), parameter :: EUP_FLAG_NO_TILE_SKIP = 1
+   integer(c_int32_t), parameter :: EUP_VTK_YZ = 0, EUP_VTK_XZ = 1, EUP_VTK_XY = 2, EUP_VTK_3D = 3
 
    ! mirrors struct eup_config (flags is uint32_t in C; same size and alignment)
    type, bind(c) :: eup_config
@@ -100,6 +101,20 @@ module eup
          type(c_ptr), value :: h
          real(c_double), intent(out) :: out(5)
       end function eup_diagno
+
+      integer(c_int) function eup_vtk_velocity(h, plane, index, out) bind(c, name='eup_vtk_velocity')
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: plane, index
+         real(c_float), intent(out) :: out(*)
+      end function eup_vtk_velocity
+
+      integer(c_int) function eup_vtk_vorticity(h, k0, out) bind(c, name='eup_vtk_vorticity')
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: k0
+         real(c_float), intent(out) :: out(*)
+      end function eup_vtk_vorticity
 
       integer(c_int32_t) function eup_abi_version() bind(c, name='eup_abi_version')
          import :: c_int32_t

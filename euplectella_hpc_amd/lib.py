@@ -11,6 +11,9 @@ import os
 
 import numpy as np
 
+from . import fortio
+from .fortio import VTK_3D, VTK_XY, VTK_XZ, VTK_YZ
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("EUP_LIBRARY") or os.path.join(HERE, "libeup.so")
 
@@ -28,7 +31,8 @@ FLAG_SHIFTED_DEVIATION = 2
 #: every function the header declares (checked by the CPU tests)
 EXPORTS = ("eup_create", "eup_destroy", "eup_last_error", "eup_set_mask", "eup_set_pops",
            "eup_get_pops", "eup_set_state", "eup_get_state", "eup_step", "eup_step_timed",
-           "eup_sync", "eup_drag_cv", "eup_drag_mem", "eup_probe", "eup_diagno", "eup_get_info",
+           "eup_sync", "eup_drag_cv", "eup_drag_mem", "eup_probe", "eup_diagno", "eup_vtk_velocity",
+           "eup_vtk_vorticity", "eup_get_info",
            "eup_stream", "eup_abi_version", "eup_set_grey", "eup_set_grey_device",
            "eup_grey_force", "eup_grey_objective", "eup_grey_objective_device", "eup_membw",
            "eup_init_equilibrium", "eup_set_omega")
@@ -97,6 +101,9 @@ def load():
     L.eup_drag_mem.argtypes = [vp, dp]
     L.eup_probe.argtypes = [vp, i32, i32, i32, dp]
     L.eup_diagno.argtypes = [vp, dp]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.eup_vtk_velocity.argtypes = [vp, i32, i32, fp]
+    L.eup_vtk_vorticity.argtypes = [vp, i32, fp]
     L.eup_get_info.argtypes = [vp, ctypes.POINTER(EupInfo)]
     L.eup_stream.argtypes = [vp]
     L.eup_stream.restype = vp
@@ -256,6 +263,54 @@ class Lattice:
         F = np.zeros(3)
         self._check(self.L.eup_drag_mem(self.h, _dptr(F)))
         return F
+
+    def vtk_velocity(self, plane, index=0):
+        """Velocity of a plane (VTK_YZ / VTK_XZ / VTK_XY at the 1-based ``index``) or of the
+        lattice (VTK_3D) as the reference's binary VTK writers compute it: float32 [sites, 3]
+        in the writer's loop order (eup_vtk_velocity)."""
+        l, m, n = self.shape
+        count = {VTK_YZ: m * n, VTK_XZ: l * n, VTK_XY: l * m, VTK_3D: l * m * n}[plane]
+        out = np.empty((count, 3), np.float32)
+        self._check(self.L.eup_vtk_velocity(self.h, int(plane), int(index),
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def vtk_vorticity(self, k0):
+        """Vorticity of the xy plane z = k0 as the reference's vtk_om_bin computes it:
+        float32 [(m-2) * (l-2)], i fastest (eup_vtk_vorticity)."""
+        l, m, n = self.shape
+        out = np.empty((m - 2) * (l - 2), np.float32)
+        self._check(self.L.eup_vtk_vorticity(self.h, int(k0),
+                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def write_vtk(self, itime, directory=".", branch="default", rank=0):
+        """Write the binary VTK files the reference writes every ``ivtim`` steps
+        (CODE/SRC/diagnostic.F90:75-90): branch "default" = velocity on the xy plane n/2-1, xz plane
+        m/2-1, yz plane l/2-1 and the vorticity of the xy plane n/2+1; "hpc" (the -DHPC build) =
+        the 3-D velocity and the xy plane n/2+1.  Returns the paths written."""
+        l, m, n = self.shape
+        shape = (l, m, n)
+        files = []
+        if branch == "hpc":
+            items = [("3d", fortio.vtk_velocity_bytes(VTK_3D, 0, shape, self.vtk_velocity(VTK_3D))),
+                     ("xy", fortio.vtk_velocity_bytes(VTK_XY, n // 2 + 1, shape,
+                                                      self.vtk_velocity(VTK_XY, n // 2 + 1)))]
+        elif branch == "default":
+            items = [("xy", fortio.vtk_velocity_bytes(VTK_XY, n // 2 - 1, shape,
+                                                      self.vtk_velocity(VTK_XY, n // 2 - 1))),
+                     ("xz", fortio.vtk_velocity_bytes(VTK_XZ, m // 2 - 1, shape,
+                                                      self.vtk_velocity(VTK_XZ, m // 2 - 1))),
+                     ("yz", fortio.vtk_velocity_bytes(VTK_YZ, l // 2 - 1, shape,
+                                                      self.vtk_velocity(VTK_YZ, l // 2 - 1))),
+                     ("om", fortio.vtk_vorticity_bytes(shape, self.vtk_vorticity(n // 2 + 1)))]
+        else:
+            raise ValueError("branch must be 'default' or 'hpc'")
+        for kind, data in items:
+            path = os.path.join(directory, fortio.vtk_file_name(kind, itime, rank))
+            fortio.write_bytes(path, data)
+            files.append(path)
+        return files
 
     def probe(self, i, j, k):
         out = np.zeros(4)

@@ -162,6 +162,26 @@ int eup_probe(eup_handle *h, int32_t i, int32_t j, int32_t k, double out[4]);
  * rho = 1, u = 0.  FP64 reduction on the GPU (fixed order; not the reference's sequential order). */
 int eup_diagno(eup_handle *h, double out[5]);
 
+/* Fields of the reference's binary VTK writers, for a host that writes those files (header and
+ * big-endian data layout: vtk_xy.binary.F90:43-72).  Velocity of a plane or of the lattice, three
+ * floats (u_x, u_y, u_z) per site in the writer's loop order:
+ *   EUP_VTK_YZ  x = index, loops k, j      (vtk_yz.binary.F90:90-118, vtk_yz_bin(itime, i0))
+ *   EUP_VTK_XZ  y = index, loops k, i      (vtk_xz.binary.F90:90-118, vtk_xz_bin(itime, j0))
+ *   EUP_VTK_XY  z = index, loops j, i      (vtk_xy.binary.F90:90-118, vtk_xy_bin(itime, k0))
+ *   EUP_VTK_3D  whole lattice, loops k, j, i; index ignored (vtk_3d.binary.F90:78-104)
+ * computed as the writers do (momenta and density summed in FP64 in their order, rounded to
+ * float, divided in float), so a parity-mode state gives the reference's files bit for bit.
+ * out holds 3 * (sites of the plane) floats; 1 <= index <= the plane's normal extent. */
+#define EUP_VTK_YZ 0
+#define EUP_VTK_XZ 1
+#define EUP_VTK_XY 2
+#define EUP_VTK_3D 3
+int eup_vtk_velocity(eup_handle *h, int32_t plane, int32_t index, float *out);
+
+/* Vorticity of the xy plane z = k0 as vtk_om.binary.F90:88-131 computes it, including the
+ * writer's evaluation order: (l-2)*(m-2) floats, i = 2..l-1 fastest, j = 2..m-1.  l, m >= 3. */
+int eup_vtk_vorticity(eup_handle *h, int32_t k0, float *out);
+
 /* --- grey lattice and adjoint (EUP_MODE_GREY; no reference counterpart) --------------------- */
 /* The link between x and its source y = x - c_q carries sigma = 1 - (1 - s(x))(1 - s(y)) and the
  * pulled population is (1 - sigma) a_q(y) + sigma a_qbar(x); s in {0, 1} is the physics mode's

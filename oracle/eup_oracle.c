@@ -616,6 +616,85 @@ void orc_diagno(const double *a, const int32_t *obs, int l, int m, int n, double
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* binary VTK output fields (reference vtk_*.binary.F90)                                       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* velocity of one site as the binary VTK writers compute it (vtk_xy.binary.F90:93-115,
+ * vtk_3d.binary.F90:80-100): the momenta and the density summed in FP64 in the written order,
+ * each rounded to real(sp), then divided in single precision; o = {u_x, u_y, u_z} */
+static void vtk_site(const double *a, geom_t g, int i, int j, int k, float o[3])
+{
+    double x[19];
+    gather(a, g, IDX(g, i, j, k), x);
+    const float u = (float)(x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]);
+    const float w = (float)(x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]);
+    const float v = (float)(x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]);
+    const float den = (float)(x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9]
+                              + x[10] + x[11] + x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18]);
+    o[0] = u / den;
+    o[1] = w / den;
+    o[2] = v / den;
+}
+
+/* plane 0: yz plane x = index (vtk_yz.binary.F90:90-118, loops k, j); 1: xz plane y = index
+ * (vtk_xz.binary.F90, loops k, i); 2: xy plane z = index (vtk_xy.binary.F90, loops j, i);
+ * 3: the whole lattice (vtk_3d.binary.F90:78-104, loops k, j, i).  out: 3 floats per site in
+ * loop order (innermost fastest). */
+void orc_vtk_velocity(const double *a, int l, int m, int n, int plane, int index, float *out)
+{
+    geom_t g = mkgeom(l, m, n);
+    long t = 0;
+    if (plane == 0) {
+        for (int k = 1; k <= n; ++k)
+            for (int j = 1; j <= m; ++j, ++t) vtk_site(a, g, index, j, k, out + 3 * t);
+    } else if (plane == 1) {
+        for (int k = 1; k <= n; ++k)
+            for (int i = 1; i <= l; ++i, ++t) vtk_site(a, g, i, index, k, out + 3 * t);
+    } else if (plane == 2) {
+        for (int j = 1; j <= m; ++j)
+            for (int i = 1; i <= l; ++i, ++t) vtk_site(a, g, i, j, index, out + 3 * t);
+    } else {
+        for (int k = 1; k <= n; ++k)
+            for (int j = 1; j <= m; ++j)
+                for (int i = 1; i <= l; ++i, ++t) vtk_site(a, g, i, j, k, out + 3 * t);
+    }
+}
+
+/* vorticity of the xy plane z = k0 as vtk_om.binary.F90:88-131 computes it, for i = 2..l-1
+ * (fastest), j = 2..m-1.  The writer's expressions are kept as written: the densities are
+ * rounded to real(sp); in each momentum only the last population is divided by the density
+ * (operator precedence), the rest of the sum is not; the difference is formed in single
+ * precision. */
+void orc_vtk_vorticity(const double *a, int l, int m, int n, int k0, float *out)
+{
+    geom_t g = mkgeom(l, m, n);
+    long t = 0;
+    double x[19];
+    for (int j = 2; j <= m - 1; ++j)
+        for (int i = 2; i <= l - 1; ++i, ++t) {
+            float d[4];
+            const int di[4] = {0, 0, -1, 1}, dj[4] = {1, -1, 0, 0}; /* up, down, left, right */
+            for (int s = 0; s < 4; ++s) {
+                gather(a, g, IDX(g, i + di[s], j + dj[s], k0), x);
+                d[s] = (float)(x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9]
+                               + x[10] + x[11] + x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18]);
+            }
+            float mom[4];
+            for (int s = 0; s < 2; ++s) { /* up1, down1: x momentum */
+                gather(a, g, IDX(g, i + di[s], j + dj[s], k0), x);
+                mom[s] = (float)(x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12]
+                                 - x[13] / (double)d[s]);
+            }
+            for (int s = 2; s < 4; ++s) { /* left1, right1: y momentum */
+                gather(a, g, IDX(g, i + di[s], j + dj[s], k0), x);
+                mom[s] = (float)(x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16]
+                                 - x[17] / (double)d[s]);
+            }
+            out[t] = 0.5f * (mom[0] - mom[1]) - 0.5f * (mom[3] - mom[2]);
+        }
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* grey lattice (partial bounce-back) and its discrete adjoint -- no reference counterpart     */
 /*                                                                                            */
 /* Link-wise partial bounce-back: the link between x and its source y = x - c_q carries the   */

@@ -36,6 +36,9 @@ def lib():
         L.orc_phase_movef.argtypes = [_dp, _dp, i, i, i]
         L.orc_phase_col.argtypes = [_dp, _dp, i, i, i, d, d]
         L.orc_collide_site.argtypes = [_dp, _dp, d, d]
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.orc_vtk_velocity.argtypes = [_dp, i, i, i, i, i, _fp]
+        L.orc_vtk_vorticity.argtypes = [_dp, i, i, i, i, _fp]
         L.orc_flux_x.argtypes = [_dp, i, i, i, d, _ip, _dp]
         L.orc_flux_y.argtypes = [_dp, i, i, i, d, _ip, _dp]
         L.orc_probe.argtypes = [_dp, i, i, i, i, i, i, _dp]
@@ -144,6 +147,30 @@ def diagno(pops, mask):
     l, m, n = _shape(pops)
     out = np.zeros(5)
     lib().orc_diagno(_d(pops), _obs(mask).ctypes.data_as(_ip), l, m, n, _d(out))
+    return out
+
+
+VTK_YZ, VTK_XZ, VTK_XY, VTK_3D = 0, 1, 2, 3
+
+
+def vtk_velocity(pops, plane, index=0):
+    """Velocity of a plane (VTK_YZ / VTK_XZ / VTK_XY at the 1-based ``index``) or of the whole
+    lattice (VTK_3D) as the reference's binary VTK writers compute it: float32 ``[..., 3]``."""
+    l, m, n = _shape(pops)
+    count = {VTK_YZ: m * n, VTK_XZ: l * n, VTK_XY: l * m, VTK_3D: l * m * n}[plane]
+    out = np.zeros((count, 3), np.float32)
+    lib().orc_vtk_velocity(_d(pops), l, m, n, int(plane), int(index),
+                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return out
+
+
+def vtk_vorticity(pops, k0):
+    """Vorticity of the xy plane z = k0 as the reference's vtk_om_bin computes it:
+    float32 ``[(m-2) * (l-2)]``, i fastest."""
+    l, m, n = _shape(pops)
+    out = np.zeros((m - 2) * (l - 2), np.float32)
+    lib().orc_vtk_vorticity(_d(pops), l, m, n, int(k0),
+                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
     return out
 
 

@@ -27,11 +27,12 @@ def ref_binary(variant="serial"):
 
 
 def run_reference(l, m, n, svisc, u0, steps, mask, pops0, *, variant="nodrag", itsave=None,
-                  icheck=None, threads=1, workdir=None, keep=False, timeout=3600):
+                  icheck=None, ivtim=None, threads=1, workdir=None, keep=False, timeout=3600):
     """Run ``steps`` time steps of the reference from the ghosted state ``pops0``.
 
     Returns a dict with ``saves`` ({itime: pops}), ``drag`` (list of (delta, itime, fx, fy)),
-    ``probe`` (list of (itime, u, v, w, rho)), ``mlups`` and ``stdout``.
+    ``probe`` (list of (itime, u, v, w, rho)), ``vtk`` ({file name: bytes} of the binary VTK
+    files written every ``ivtim`` steps, CODE/SRC/diagnostic.F90:40-90), ``mlups`` and ``stdout``.
     """
     exe = ref_binary(variant)
     if exe is None:
@@ -41,7 +42,7 @@ def run_reference(l, m, n, svisc, u0, steps, mask, pops0, *, variant="nodrag", i
     try:
         big = steps + 10
         fortio.write_deck(os.path.join(wd, "bgk.input"), svisc=float(svisc), u0=float(u0),
-                          itfin=int(steps), ivtim=big, isignal=big,
+                          itfin=int(steps), ivtim=int(ivtim if ivtim else big), isignal=big,
                           itsave=int(itsave if itsave else big), icheck=int(icheck if icheck else big),
                           irestart=1, init_v=0, lx=l, ly=m, lz=n, proc_x=1, proc_y=1, proc_z=1,
                           tstep=1, flag1=1, flag2=2, flag3=0)
@@ -62,6 +63,11 @@ def run_reference(l, m, n, svisc, u0, steps, mask, pops0, *, variant="nodrag", i
             if mt:
                 saves[int(mt.group(1))] = fortio.read_raw(os.path.join(wd, name), (n, m, l))[1]
         out["saves"] = saves
+        out["vtk"] = {}
+        for name in sorted(os.listdir(wd)):
+            if re.match(r"tec_(3d|xy|xz|yz|om)\.\d{4}\.\d{8}\.vtk$", name):
+                with open(os.path.join(wd, name), "rb") as fh:
+                    out["vtk"][name] = fh.read()
         out["drag"] = _read_table(os.path.join(wd, "drag.0000.dat"), 5)
         out["probe"] = _read_table(os.path.join(wd, "probe_g.0000.dat"), 5)
         out["drag_text"] = _read_text(os.path.join(wd, "drag.0000.dat"))
@@ -121,7 +127,7 @@ def time_reference(l, m, n, svisc, u0, steps, mask, *, threads=None, variant="om
     try:
         big = steps + 10
         fortio.write_deck(os.path.join(wd, "bgk.input"), svisc=float(svisc), u0=float(u0),
-                          itfin=int(steps), ivtim=big, isignal=big, itsave=big, icheck=big,
+                          itfin=int(steps), ivtim=int(ivtim if ivtim else big), isignal=big, itsave=big, icheck=big,
                           irestart=0, init_v=0, lx=l, ly=m, lz=n, proc_x=1, proc_y=1, proc_z=1,
                           tstep=1, flag1=1, flag2=2, flag3=0)
         fortio.write_obs(os.path.join(wd, fortio.obs_file_name()), mask)

@@ -8,7 +8,9 @@ on inputs written by this repository's seeded generators, and records:
 * SHA-256 of the full ghosted population state saved by the reference (save_raw) after given
   step counts, and for the smallest case the whole final state (``*.npz``);
 * the reference's probe records (probe_g.0000.dat, e14.6 text) and drag records (drag.0000.dat,
-  17-digit doubles) where the case enables them.
+  17-digit doubles) where the case enables them;
+* SHA-256 and size of the binary VTK files (tec_*.vtk) written at the last step by the default
+  output branch and by the -DHPC build (CODE/SRC/diagnostic.F90:75-90) where the case enables them.
 
 Usage (in a container where /root/reference is present):
     make -f oracle/Makefile.ref && python tests/golden/make_golden.py
@@ -32,9 +34,9 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 CASES = {
     "c0_tiny": dict(l=16, m=8, n=12, geom=("cylinder", dict(radius=2.5, centre=(5, 4.5))),
                     tau=0.6, u=0.05, seed=11, steps=20, saves=(10, 20), probe=True,
-                    full_state=True),
+                    full_state=True, vtk=True),
     "c1_small": dict(l=32, m=16, n=16, geom=("cylinder", dict(radius=4, centre=(8, 8))),
-                     tau=0.6, u=0.05, seed=1, steps=10, saves=(5, 10), probe=True),
+                     tau=0.6, u=0.05, seed=1, steps=10, saves=(5, 10), probe=True, vtk=True),
     "c1_cylinder": dict(l=128, m=64, n=64, geom=("cylinder", dict(radius=8, centre=(32, 32))),
                         tau=0.6, u=0.05, seed=2024, steps=2000, saves=(200, 1000, 2000),
                         probe=True),
@@ -74,9 +76,11 @@ def main(names=None):
         drag = case.get("drag", False)
         saves = case["saves"]
         step = saves[0] if len(saves) == 1 else int(np.gcd.reduce(np.array(saves)))
+        vtk = case.get("vtk", False)
         res = refrun.run_reference(l, m, n, svisc, case["u"], case["steps"], mask, pops,
                                    variant="serial" if drag else "nodrag", itsave=step,
-                                   icheck=1 if (drag or case.get("probe")) else None)
+                                   icheck=1 if (drag or case.get("probe")) else None,
+                                   ivtim=case["steps"] if vtk else None)
         rec = {
             "l": l, "m": m, "n": n, "geometry": [case["geom"][0], case["geom"][1]],
             "tau": case["tau"], "svisc": svisc, "u_in": case["u"], "seed": case["seed"],
@@ -92,6 +96,14 @@ def main(names=None):
         if drag:
             rec["drag"] = [list(r[:4]) for r in res["drag"]]
             rec["drag_text"] = res["drag_text"]
+        if vtk:
+            hpc = refrun.run_reference(l, m, n, svisc, case["u"], case["steps"], mask, pops,
+                                       variant="hpc", itsave=step, ivtim=case["steps"])
+            assert sha(hpc["saves"][saves[-1]]) == rec["state_sha256"][str(saves[-1])]
+            rec["vtk"] = {branch: {name: {"sha256": hashlib.sha256(data).hexdigest(),
+                                          "bytes": len(data)}
+                                   for name, data in r["vtk"].items()}
+                          for branch, r in (("default", res), ("hpc", hpc))}
         if case.get("full_state"):
             last = saves[-1]
             np.savez_compressed(os.path.join(OUT, f"{name}_state.npz"),

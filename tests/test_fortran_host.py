@@ -19,9 +19,9 @@ from golden_cases import CASES, make_inputs
 EXE = eup_build.FORTRAN_EXE
 
 
-def write_case(wd, rec, mask, pops, icheck, itsave, irestart=1):
+def write_case(wd, rec, mask, pops, icheck, itsave, irestart=1, ivtim=None):
     fortio.write_deck(os.path.join(wd, "bgk.input"), svisc=rec["svisc"], u0=rec["u_in"],
-                      itfin=rec["steps"], ivtim=rec["steps"] + 10, isignal=rec["steps"] + 10,
+                      itfin=rec["steps"], ivtim=ivtim or rec["steps"] + 10, isignal=rec["steps"] + 10,
                       itsave=itsave, icheck=icheck, irestart=irestart, init_v=0,
                       lx=rec["l"], ly=rec["m"], lz=rec["n"], proc_x=1, proc_y=1, proc_z=1,
                       tstep=1, flag1=1, flag2=2, flag3=0)
@@ -68,6 +68,26 @@ def test_host_parity_run_matches_reference_files(golden):
                                        (rec["n"], rec["m"], rec["l"]))
             digest = hashlib.sha256(np.ascontiguousarray(state).tobytes()).hexdigest()
             assert digest == rec["state_sha256"][str(t)], f"save dump at step {t} differs"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("branch", ["default", "hpc"])
+def test_host_vtk_files_match_reference(golden, branch):
+    """ivtim = the run length: the host writes the reference's binary VTK files of the chosen
+    output branch, byte for byte."""
+    rec = golden["c1_small"]
+    mask, pops = make_inputs(CASES["c1_small"])
+    print(f"[gpu] Fortran host, parity mode, c1_small VTK branch {branch}", flush=True)
+    with tempfile.TemporaryDirectory() as wd:
+        write_case(wd, rec, mask, pops, icheck=rec["steps"], itsave=rec["steps"], ivtim=rec["steps"])
+        res = run_host(wd, {"EUP_MODE": "parity", "EUP_VTK": branch})
+        assert res.returncode == 0, res.stdout + res.stderr
+        got = sorted(f for f in os.listdir(wd) if f.startswith("tec_"))
+        want = rec["vtk"][branch]
+        assert got == sorted(want)
+        for name in got:
+            with open(os.path.join(wd, name), "rb") as fh:
+                assert hashlib.sha256(fh.read()).hexdigest() == want[name]["sha256"], name
 
 
 @pytest.mark.gpu

@@ -91,6 +91,26 @@ def test_parity_drag_records(golden):
         assert sha(lat.get_state()) == rec["state_sha256"][str(rec["steps"])]
 
 
+@pytest.mark.parametrize("name", ["c0_tiny", "c1_small"])
+def test_parity_vtk_files_match_reference(golden, name, tmp_path):
+    """eup_vtk_velocity / eup_vtk_vorticity on the parity state: the binary VTK files of both
+    output branches equal the reference's byte for byte."""
+    import hashlib
+    rec, mask, pops, lat = parity_lattice(name, golden)
+    announce(f"parity {name}: VTK files after {rec['steps']} steps")
+    with lat:
+        lat.step(rec["steps"])
+        for branch in ("default", "hpc"):
+            d = tmp_path / branch
+            d.mkdir()
+            paths = lat.write_vtk(rec["steps"], str(d), branch)
+            want = rec["vtk"][branch]
+            assert sorted(os.path.basename(p) for p in paths) == sorted(want)
+            for p in paths:
+                data = open(p, "rb").read()
+                assert hashlib.sha256(data).hexdigest() == want[os.path.basename(p)]["sha256"], p
+
+
 # ----------------------------------------------------------------------------------------------
 # physics mode
 # ----------------------------------------------------------------------------------------------
@@ -199,6 +219,29 @@ def test_physics_drag_mem_and_diagno_match_oracle():
     assert np.allclose(d, dr, rtol=1e-13, atol=1e-16), (d, dr)
     pr = orc.probe(ref, 40, 20, 5)
     assert np.array_equal(p, pr)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f16"])
+def test_physics_vtk_fields_equal_oracle(storage):
+    """VTK velocity planes, volume and vorticity of a physics-mode state (tile-major layout, any
+    storage) equal the oracle's on the same state read back as FP64."""
+    l, m, n = 70, 24, 20
+    announce(f"physics {storage} sponge {l}x{m}x{n}: VTK fields vs oracle")
+    mask, pops = physics_case("sponge", l, m, n, seed=12)
+    with lattice((l, m, n), svisc=(2 * 0.6 - 1) / 6, u_inflow=0.05, mode="physics",
+                 storage=storage) as lat:
+        lat.set_mask(mask)
+        lat.set_state(pops)
+        lat.step(15)
+        state = lat.get_state()
+        for plane, index in ((orc.VTK_YZ, 1), (orc.VTK_YZ, l), (orc.VTK_XZ, 7), (orc.VTK_XY, n),
+                             (orc.VTK_3D, 0)):
+            got = lat.vtk_velocity(plane, index)
+            assert np.array_equal(got.view(np.uint32),
+                                  orc.vtk_velocity(state, plane, index).view(np.uint32)), (plane, index)
+        for k0 in (1, 11):
+            assert np.array_equal(lat.vtk_vorticity(k0).view(np.uint32),
+                                  orc.vtk_vorticity(state, k0).view(np.uint32)), k0
 
 
 def test_momentum_exchange_closes_the_momentum_balance():

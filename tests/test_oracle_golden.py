@@ -68,3 +68,38 @@ def test_oracle_drag_records(golden):
             assert orc.flux_x(pops, om, box) == fx
             assert orc.flux_y(pops, om, box) == fy
     assert sha(pops) == rec["state_sha256"][str(rec["steps"])]
+
+
+def vtk_files(shape, velocity, vorticity, branch, itime):
+    """{file name: bytes} of one output branch of the reference (CODE/SRC/diagnostic.F90:75-90),
+    from field functions velocity(plane, index) and vorticity(k0)."""
+    from euplectella_hpc_amd import fortio
+    l, m, n = shape
+    if branch == "hpc":
+        planes = [("3d", fortio.VTK_3D, 0), ("xy", fortio.VTK_XY, n // 2 + 1)]
+    else:
+        planes = [("xy", fortio.VTK_XY, n // 2 - 1), ("xz", fortio.VTK_XZ, m // 2 - 1),
+                  ("yz", fortio.VTK_YZ, l // 2 - 1)]
+    out = {fortio.vtk_file_name(kind, itime): fortio.vtk_velocity_bytes(plane, index, shape,
+                                                                        velocity(plane, index))
+           for kind, plane, index in planes}
+    if branch == "default":
+        out[fortio.vtk_file_name("om", itime)] = fortio.vtk_vorticity_bytes(shape, vorticity(n // 2 + 1))
+    return out
+
+
+@pytest.mark.parametrize("branch", ["default", "hpc"])
+def test_oracle_vtk_files_match_reference(golden, branch):
+    """orc_vtk_velocity / orc_vtk_vorticity and the fortio writer, applied to the reference's own
+    saved state, give the reference's binary VTK files byte for byte."""
+    import hashlib
+    rec = golden["c0_tiny"]
+    state = np.load(os.path.join(GOLDEN, "c0_tiny_state.npz"))[f"t{rec['steps']}"]
+    files = vtk_files((rec["l"], rec["m"], rec["n"]),
+                      lambda p, i: orc.vtk_velocity(state, p, i),
+                      lambda k: orc.vtk_vorticity(state, k), branch, rec["steps"])
+    want = rec["vtk"][branch]
+    assert sorted(files) == sorted(want)
+    for name, data in files.items():
+        assert len(data) == want[name]["bytes"], name
+        assert hashlib.sha256(data).hexdigest() == want[name]["sha256"], name
