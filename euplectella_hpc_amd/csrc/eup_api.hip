@@ -5,6 +5,9 @@
 #include <new>
 #include <string>
 
+#include <algorithm>
+#include <climits>
+
 #include "eup_internal.h"
 
 namespace eup {
@@ -99,7 +102,7 @@ static void release(Handle *h)
 {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
-    void *ptrs[] = {h->alloc[0], h->alloc[1], h->staging, h->obs, h->solid, h->solid_list, h->bb_tmp,
+    void *ptrs[] = {h->alloc[0], h->alloc[1], h->staging, h->obs_stage, h->solid, h->solid_list, h->bb_tmp,
                     h->tile_code, h->link_mask, h->mixed_list, h->outlet[0], h->outlet[1], h->diag,
                     h->site19, h->grey};
     for (void *p : ptrs)
@@ -108,6 +111,61 @@ static void release(Handle *h)
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);
 }
+
+namespace eup {
+namespace {
+
+// sites of one chunk of the obstacle list marked solid (indices validated on the host)
+__global__ void k_mark_solid(const int32_t *__restrict__ ijk, int64_t count, int l, int m,
+                             uint8_t *__restrict__ solid)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t i = ijk[3 * t] - 1, j = ijk[3 * t + 1] - 1, k = ijk[3 * t + 2] - 1;
+    solid[i + (int64_t)l * (j + (int64_t)m * k)] = 1;
+}
+
+constexpr int kStatBlocks = 1024;
+
+// number of solid sites and their bounding box (1-based, min/max per axis), per block
+__global__ __launch_bounds__(256) void k_mask_stats(const uint8_t *__restrict__ solid, int l, int m,
+                                                    int64_t sites, int64_t *__restrict__ part)
+{
+    __shared__ int64_t sc[256];
+    __shared__ int sb[6][256];
+    int64_t c = 0;
+    int b[6] = {l + 1, 0, m + 1, 0, INT32_MAX, 0};
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < sites; s += (int64_t)gridDim.x * blockDim.x) {
+        if (!solid[s]) continue;
+        const int i = 1 + (int)(s % l);
+        const int64_t r = s / l;
+        const int j = 1 + (int)(r % m), k = 1 + (int)(r / m);
+        ++c;
+        b[0] = min(b[0], i); b[1] = max(b[1], i);
+        b[2] = min(b[2], j); b[3] = max(b[3], j);
+        b[4] = min(b[4], k); b[5] = max(b[5], k);
+    }
+    sc[threadIdx.x] = c;
+    for (int a = 0; a < 6; ++a) sb[a][threadIdx.x] = b[a];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sc[threadIdx.x] += sc[threadIdx.x + w];
+            for (int a = 0; a < 6; a += 2) {
+                sb[a][threadIdx.x] = min(sb[a][threadIdx.x], sb[a][threadIdx.x + w]);
+                sb[a + 1][threadIdx.x] = max(sb[a + 1][threadIdx.x], sb[a + 1][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[8 * blockIdx.x] = sc[0];
+        for (int a = 0; a < 6; ++a) part[8 * blockIdx.x + 1 + a] = sb[a][0];
+    }
+}
+
+} // namespace
+} // namespace eup
 
 extern "C" {
 
@@ -164,7 +222,6 @@ int eup_create(const eup_config *cfg, eup_handle **out)
     for (int b = 0; b < 2 && !rc && h->tiled; ++b)
         if (hip_check(h, hipMemset(h->alloc[b], 0, state), "hipMemset populations")) rc = EUP_EHIP;
     if (!rc && (hip_check(h, hipMalloc((void **)&h->staging, sizeof(double) * g.ghosted), "hipMalloc staging") ||
-                hip_check(h, hipMalloc((void **)&h->obs, sizeof(int32_t) * g.sites), "hipMalloc obs") ||
                 hip_check(h, hipMalloc((void **)&h->solid, g.sites), "hipMalloc solid")))
         rc = EUP_EHIP;
     if (!rc && cfg->mode == EUP_MODE_PHYSICS) {
@@ -181,7 +238,7 @@ int eup_create(const eup_config *cfg, eup_handle **out)
         delete h;
         return rc;
     }
-    h->device_bytes = (int64_t)(2 * state + sizeof(double) * g.ghosted + 5 * g.sites);
+    h->device_bytes = (int64_t)(2 * state + sizeof(double) * g.ghosted + g.sites);
     *out = h;
     return 0;
 }
@@ -227,8 +284,7 @@ int eup_set_mask(eup_handle *h, const int32_t *obs)
                 }
             }
     int rc = 0;
-    if (hip_check(h, hipMemcpy(h->obs, obs, sizeof(int32_t) * g.sites, hipMemcpyHostToDevice), "copy obs") ||
-        hip_check(h, hipMemcpy(h->solid, sol, g.sites, hipMemcpyHostToDevice), "copy solid"))
+    if (hip_check(h, hipMemcpy(h->solid, sol, g.sites, hipMemcpyHostToDevice), "copy solid"))
         rc = EUP_EHIP;
     hipHostFree(sol);
     if (rc) return rc;
@@ -236,6 +292,94 @@ int eup_set_mask(eup_handle *h, const int32_t *obs)
     memcpy(h->bbox, bb, sizeof bb);
     rc = h->cfg.mode == EUP_MODE_PARITY ? parity_prepare_mask(h) : physics_prepare_mask(h);
     if (rc) return rc;
+    h->obstacles_open = false;
+    h->mask_set = true;
+    return 0;
+}
+
+int eup_add_obstacles(eup_handle *h, const int32_t *ijk, int64_t count)
+{
+    if (!h || count < 0 || (count > 0 && !ijk)) return fail(h, EUP_EINVAL, "eup_add_obstacles: bad argument");
+    if (h->cfg.mode == EUP_MODE_GREY)
+        return fail(h, EUP_EINVAL, "eup_add_obstacles: grey mode takes a solid fraction (eup_set_grey)");
+    if (hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice")) return EUP_EHIP;
+    const Geometry &g = h->g;
+    // the whole list is checked before anything changes, so a rejected call leaves the handle as
+    // it was
+    for (int64_t t = 0; t < count; ++t) {
+        const int32_t *p = ijk + 3 * t;
+        if (p[0] < 1 || p[0] > g.l || p[1] < 1 || p[1] > g.m || p[2] < 1 || p[2] > g.n)
+            return fail(h, EUP_ERANGE, "eup_add_obstacles: obstacle site outside the lattice");
+    }
+    if (!h->obstacles_open) {
+        if (hip_check(h, hipMemsetAsync(h->solid, 0, g.sites, h->stream), "hipMemset solid")) return EUP_EHIP;
+        h->obstacles_open = true;
+        h->mask_set = false;
+    }
+    constexpr int64_t kChunk = 1 << 20; // triplets per staged chunk (12 MiB)
+    if (!h->obs_stage &&
+        hip_check(h, hipMalloc((void **)&h->obs_stage, sizeof(int32_t) * 3 * kChunk), "hipMalloc obstacle stage"))
+        return EUP_EHIP;
+    for (int64_t c0 = 0; c0 < count; c0 += kChunk) {
+        const int64_t c = count - c0 < kChunk ? count - c0 : kChunk;
+        const int32_t *p = ijk + 3 * c0;
+        // the chunk is staged and marked on the handle's stream; the host buffer and the stage are
+        // reused only after the stream has drained
+        if (hip_check(h, hipMemcpyAsync(h->obs_stage, p, sizeof(int32_t) * 3 * c, hipMemcpyHostToDevice, h->stream),
+                      "copy obstacles"))
+            return EUP_EHIP;
+        hipLaunchKernelGGL(k_mark_solid, dim3((unsigned)((c + 255) / 256)), dim3(256), 0, h->stream, h->obs_stage, c,
+                           g.l, g.m, h->solid);
+        if (hip_check(h, hipGetLastError(), "mark obstacles") || hip_check(h, hipStreamSynchronize(h->stream), "sync"))
+            return EUP_EHIP;
+    }
+    return 0;
+}
+
+int eup_commit_obstacles(eup_handle *h)
+{
+    if (!h) return fail(nullptr, EUP_EINVAL, "eup_commit_obstacles: null handle");
+    if (h->cfg.mode == EUP_MODE_GREY)
+        return fail(h, EUP_EINVAL, "eup_commit_obstacles: grey mode takes a solid fraction (eup_set_grey)");
+    if (hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice")) return EUP_EHIP;
+    const Geometry &g = h->g;
+    if (!h->obstacles_open && hip_check(h, hipMemsetAsync(h->solid, 0, g.sites, h->stream), "hipMemset solid"))
+        return EUP_EHIP;
+    int64_t *part = nullptr, *part_h = nullptr;
+    int rc = 0;
+    if (hip_check(h, hipMalloc((void **)&part, sizeof(int64_t) * 8 * kStatBlocks), "hipMalloc") ||
+        hip_check(h, hipHostMalloc((void **)&part_h, sizeof(int64_t) * 8 * kStatBlocks), "hipHostMalloc"))
+        rc = EUP_EHIP;
+    if (!rc) {
+        hipLaunchKernelGGL(k_mask_stats, dim3(kStatBlocks), dim3(256), 0, h->stream, h->solid, g.l, g.m, g.sites,
+                           part);
+        if (hip_check(h, hipGetLastError(), "mask stats") ||
+            hip_check(h, hipMemcpyAsync(part_h, part, sizeof(int64_t) * 8 * kStatBlocks, hipMemcpyDeviceToHost,
+                                        h->stream), "copy stats") ||
+            hip_check(h, hipStreamSynchronize(h->stream), "sync"))
+            rc = EUP_EHIP;
+    }
+    if (!rc) {
+        // the bounding box starts from (l, 0, m, 0, n, 0), as the reference's reader does
+        int64_t ns = 0;
+        int bb[6] = {g.l, 0, g.m, 0, g.n, 0};
+        for (int b = 0; b < kStatBlocks; ++b) {
+            const int64_t *q = part_h + 8 * b;
+            if (q[0] == 0) continue;
+            ns += q[0];
+            bb[0] = std::min(bb[0], (int)q[1]); bb[1] = std::max(bb[1], (int)q[2]);
+            bb[2] = std::min(bb[2], (int)q[3]); bb[3] = std::max(bb[3], (int)q[4]);
+            bb[4] = std::min(bb[4], (int)q[5]); bb[5] = std::max(bb[5], (int)q[6]);
+        }
+        h->nsolid = ns;
+        memcpy(h->bbox, bb, sizeof bb);
+    }
+    hipFree(part);
+    if (part_h) hipHostFree(part_h);
+    if (rc) return rc;
+    rc = h->cfg.mode == EUP_MODE_PARITY ? parity_prepare_mask(h) : physics_prepare_mask(h);
+    if (rc) return rc;
+    h->obstacles_open = false;
     h->mask_set = true;
     return 0;
 }
@@ -315,7 +459,7 @@ static int ready(eup_handle *h, const char *who)
         if (!h->grey_set)
             return fail(h, EUP_ESTATE, std::string(who) + ": eup_set_grey has not been called");
     } else if (!h->mask_set) {
-        return fail(h, EUP_ESTATE, std::string(who) + ": eup_set_mask has not been called");
+        return fail(h, EUP_ESTATE, std::string(who) + ": no obstacle mask (eup_set_mask or eup_commit_obstacles)");
     }
     if (h->pops_set != 0x7FFFFu)
         return fail(h, EUP_ESTATE, std::string(who) + ": not all 19 populations have been set");

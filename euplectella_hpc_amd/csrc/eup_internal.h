@@ -43,8 +43,9 @@ struct Handle {
     bool mask_set = false;
     uint32_t pops_set = 0;        // bit q-1 set once population q has been provided
     // mask derived data
-    int32_t *obs = nullptr;       // device copy of obs (l,m,n)
     uint8_t *solid = nullptr;     // dense solid flags (l*m*n)
+    bool obstacles_open = false;  // eup_add_obstacles called since the last commit
+    int32_t *obs_stage = nullptr; // device staging of one chunk of obstacle triplets
     int64_t nsolid = 0;
     int32_t bbox[6] = {0, 0, 0, 0, 0, 0};
     // parity mode

@@ -222,20 +222,17 @@ int parity_prepare_mask(Handle *h)
 {
     // solid list in the reference's bbox loop order (k, j, i), as ghosted offsets
     const Geometry &g = h->g;
-    std::string dummy;
-    int32_t *obs_h = nullptr;
-    if (hip_check(h, hipHostMalloc((void **)&obs_h, sizeof(int32_t) * g.sites), "hipHostMalloc"))
-        return EUP_EHIP;
-    if (hip_check(h, hipMemcpy(obs_h, h->obs, sizeof(int32_t) * g.sites, hipMemcpyDeviceToHost),
-                  "hipMemcpy obs")) {
-        hipHostFree(obs_h);
+    uint8_t *sol_h = nullptr;
+    if (hip_check(h, hipHostMalloc((void **)&sol_h, g.sites), "hipHostMalloc")) return EUP_EHIP;
+    if (hip_check(h, hipMemcpy(sol_h, h->solid, g.sites, hipMemcpyDeviceToHost), "hipMemcpy solid")) {
+        hipHostFree(sol_h);
         return EUP_EHIP;
     }
     int64_t cnt = 0;
-    for (int64_t s = 0; s < g.sites; ++s) cnt += obs_h[s] == 1;
+    for (int64_t s = 0; s < g.sites; ++s) cnt += sol_h[s] != 0;
     int32_t *list_h = nullptr;
     if (cnt > 0 && hip_check(h, hipHostMalloc((void **)&list_h, sizeof(int32_t) * cnt), "hipHostMalloc")) {
-        hipHostFree(obs_h);
+        hipHostFree(sol_h);
         return EUP_EHIP;
     }
     int64_t c = 0;
@@ -243,9 +240,9 @@ int parity_prepare_mask(Handle *h)
         for (int j = 1; j <= g.m; ++j)
             for (int i = 1; i <= g.l; ++i) {
                 int64_t o = (int64_t)(i - 1) + (int64_t)g.l * ((int64_t)(j - 1) + (int64_t)g.m * (k - 1));
-                if (obs_h[o] == 1) list_h[c++] = (int32_t)ghosted_index(g, i, j, k);
+                if (sol_h[o] != 0) list_h[c++] = (int32_t)ghosted_index(g, i, j, k);
             }
-    hipHostFree(obs_h);
+    hipHostFree(sol_h);
     hipFree(h->solid_list);
     hipFree(h->bb_tmp);
     h->solid_list = nullptr;

@@ -74,6 +74,29 @@ def write_obs(path, mask):
     write_records(path, recs)
 
 
+def iter_obs(path, chunk=1 << 20):
+    """The (i, j, k) records of an obstacle file in blocks of at most ``chunk`` rows (int32
+    [rows, 3], 1-based), read sequentially without holding the whole list."""
+    with open(path, "rb") as fh:
+        head = fh.read(12)
+        if len(head) < 12:
+            raise ValueError(f"{path}: truncated obstacle file")
+        n1, nobs, n2 = struct.unpack("<iii", head)
+        if n1 != 4 or n2 != 4:
+            raise ValueError(f"{path}: bad record header")
+        done = 0
+        while done < nobs:
+            rows = min(chunk, nobs - done)
+            raw = np.frombuffer(fh.read(20 * rows), dtype="<i4")
+            if raw.size != 5 * rows:
+                raise ValueError(f"{path}: truncated obstacle file")
+            rec = raw.reshape(rows, 5)
+            if np.any(rec[:, 0] != 12) or np.any(rec[:, 4] != 12):
+                raise ValueError(f"{path}: bad record marker")
+            yield np.ascontiguousarray(rec[:, 1:4])
+            done += rows
+
+
 def read_obs(path, shape):
     """Read an obstacle list into a mask ``[n, m, l]`` (uint8)."""
     n, m, l = shape

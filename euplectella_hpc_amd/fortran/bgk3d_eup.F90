@@ -33,11 +33,11 @@ program bgk3d_eup
 
    type(eup_config) :: cfg
    type(c_ptr) :: h
-   integer(c_int32_t), allocatable :: obs(:, :, :)
+   integer(c_int32_t), allocatable :: chunk(:, :)
    real(c_double), allocatable :: buf(:)
    real(c_double) :: pr(4), dg(5), fx, fy, tloop
    integer(c_int32_t) :: box(6)
-   integer :: l, m, n, q, itime, itstart, nobs, idx, i, j, k, nxt, delta
+   integer :: l, m, n, q, itime, itstart, nobs, idx, i, j, k, nxt, delta, cnt, r
    integer(8) :: c0, c1, crate
    integer :: do_drag, vtk_branch
    integer(c_int) :: rc
@@ -92,23 +92,30 @@ program bgk3d_eup
    rc = eup_create(cfg, h)
    call eup_check(rc, c_null_ptr, 'eup_create')
 
-   ! obstacles (build_obs, flag1 = 1 reads the task's list; 0 = no obstacle)
-   allocate(obs(l, m, n))
-   obs = 0
+   ! obstacles (build_obs, flag1 = 1 reads the task's list; 0 = no obstacle): the list is streamed
+   ! to the device in chunks of at most 2^20 records, no dense obs array on the host
    if (flag1 == 1) then
       open(53, file='obs.0000.00000000.bin', form='unformatted', status='old')
       read(53) nobs
-      do idx = 1, nobs
-         read(53) i, j, k
-         obs(i, j, k) = 1
+      allocate(chunk(3, max(1, min(nobs, 1048576))))
+      idx = 0
+      do while (idx < nobs)
+         cnt = min(size(chunk, 2), nobs - idx)
+         do r = 1, cnt
+            read(53) chunk(1, r), chunk(2, r), chunk(3, r)
+         end do
+         rc = eup_add_obstacles(h, chunk, int(cnt, c_int64_t))
+         call eup_check(rc, h, 'eup_add_obstacles')
+         idx = idx + cnt
       end do
       close(53)
+      deallocate(chunk)
    else if (flag1 /= 0) then
       write(6,*) "ERROR: flag1 must be 0 or 1 (obstacle list file)", flag1
       stop 1
    end if
-   rc = eup_set_mask(h, obs)
-   call eup_check(rc, h, 'eup_set_mask')
+   rc = eup_commit_obstacles(h)
+   call eup_check(rc, h, 'eup_commit_obstacles')
 
    ! populations: restore_raw or the rest state of init opt 0
    allocate(buf((l + 2) * (m + 2) * (n + 2)))

@@ -49,6 +49,18 @@ module eup
          integer(c_int32_t), intent(in) :: obs(*)
       end function eup_set_mask
 
+      integer(c_int) function eup_add_obstacles(h, ijk, count) bind(c, name='eup_add_obstacles')
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), intent(in) :: ijk(*)
+         integer(c_int64_t), value :: count
+      end function eup_add_obstacles
+
+      integer(c_int) function eup_commit_obstacles(h) bind(c, name='eup_commit_obstacles')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function eup_commit_obstacles
+
       integer(c_int) function eup_set_pops(h, q, a) bind(c, name='eup_set_pops')
          import :: c_int, c_ptr, c_int32_t, c_double
          type(c_ptr), value :: h

@@ -35,7 +35,7 @@ EXPORTS = ("eup_create", "eup_destroy", "eup_last_error", "eup_set_mask", "eup_s
            "eup_vtk_vorticity", "eup_get_info",
            "eup_stream", "eup_abi_version", "eup_set_grey", "eup_set_grey_device",
            "eup_grey_force", "eup_grey_objective", "eup_grey_objective_device", "eup_membw",
-           "eup_init_equilibrium", "eup_set_omega")
+           "eup_init_equilibrium", "eup_set_omega", "eup_add_obstacles", "eup_commit_obstacles")
 
 
 class EupConfig(ctypes.Structure):
@@ -90,6 +90,8 @@ def load():
     L.eup_last_error.argtypes = [vp]
     L.eup_last_error.restype = ctypes.c_char_p
     L.eup_set_mask.argtypes = [vp, ip]
+    L.eup_add_obstacles.argtypes = [vp, ip, ctypes.c_int64]
+    L.eup_commit_obstacles.argtypes = [vp]
     L.eup_set_pops.argtypes = [vp, i32, dp]
     L.eup_get_pops.argtypes = [vp, i32, dp]
     L.eup_set_state.argtypes = [vp, dp]
@@ -200,6 +202,25 @@ class Lattice:
         if obs.shape != (n, m, l):
             raise ValueError(f"mask shape {obs.shape} != {(n, m, l)}")
         self._check(self.L.eup_set_mask(self.h, obs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+
+    def add_obstacles(self, ijk):
+        """Mark the sites ``ijk`` (int array [count, 3] of 1-based (i, j, k), the records of the
+        obstacle file) solid; call commit_obstacles() when the list is complete
+        (eup_add_obstacles)."""
+        a = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        self._check(self.L.eup_add_obstacles(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                             int(a.shape[0])))
+
+    def commit_obstacles(self):
+        """Finish the obstacle list given to add_obstacles (eup_commit_obstacles)."""
+        self._check(self.L.eup_commit_obstacles(self.h))
+
+    def set_obstacle_file(self, path, chunk=1 << 20):
+        """Stream an obstacle file (obs.RRRR.TTTTTTTT.bin) into the lattice, ``chunk`` records at
+        a time, without a dense mask on the host."""
+        for block in fortio.iter_obs(path, chunk):
+            self.add_obstacles(block)
+        self.commit_obstacles()
 
     def set_state(self, pops):
         l, m, n = self.shape

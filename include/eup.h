@@ -107,6 +107,18 @@ const char *eup_last_error(const eup_handle *h);
  * (build.obs.F90:147-186).  Computes the obstacle bounding box and the tile classes. */
 int eup_set_mask(eup_handle *h, const int32_t *obs);
 
+/* The obstacle list itself, streamed in chunks, without a dense host array: the file
+ * obs.RRRR.TTTTTTTT.bin holds nobs records of (i, j, k), 1-based (read by build.obs.F90:147-186,
+ * which fills obs and the bounding box while reading).  eup_add_obstacles marks count sites
+ * ijk[3*t .. 3*t+2] solid; the first call after eup_create, eup_set_mask or a commit starts from
+ * an all-fluid lattice, later calls add to it (sites may repeat).  eup_commit_obstacles then counts
+ * the solid sites and their bounding box on the GPU and prepares the mode's mask data, as
+ * eup_set_mask does; a commit without any add gives a lattice without obstacles.  Until the
+ * commit the handle has no mask (eup_step fails with EUP_ESTATE).  A list with a site outside the
+ * lattice is rejected (EUP_ERANGE) before anything changes.  Not for grey mode. */
+int eup_add_obstacles(eup_handle *h, const int32_t *ijk, int64_t count);
+int eup_commit_obstacles(eup_handle *h);
+
 /* Population q (1..19) as a ghosted FP64 host array: the record restore_raw reads
  * (restore_raw.F90:43-63).  In physics mode only the interior is used. */
 int eup_set_pops(eup_handle *h, int32_t q, const double *a);

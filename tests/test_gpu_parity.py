@@ -244,6 +244,57 @@ def test_physics_vtk_fields_equal_oracle(storage):
                                   orc.vtk_vorticity(state, k0).view(np.uint32)), k0
 
 
+@pytest.mark.parametrize("mode,storage", [("parity", "f64"), ("physics", "f64"), ("physics", "f16")])
+def test_streamed_obstacle_list_equals_dense_mask(mode, storage, tmp_path):
+    """eup_add_obstacles (chunks of 97 records, one chunk given twice) + eup_commit_obstacles give
+    the same solid count, bounding box and states as eup_set_mask with the dense array."""
+    from euplectella_hpc_amd import fortio
+    l, m, n = 70, 24, 20
+    announce(f"{mode} {storage} sponge {l}x{m}x{n}: streamed obstacle list vs dense mask")
+    mask, pops = physics_case("sponge", l, m, n, seed=21)
+    path = str(tmp_path / fortio.obs_file_name())
+    fortio.write_obs(path, mask)
+    out, infos = [], []
+    for streamed in (False, True):
+        with lattice((l, m, n), svisc=(2 * 0.6 - 1) / 6, u_inflow=0.05, mode=mode,
+                     storage=storage) as lat:
+            if streamed:
+                blocks = list(fortio.iter_obs(path, 97))
+                for b in blocks + blocks[:1]:
+                    lat.add_obstacles(b)
+                lat.commit_obstacles()
+            else:
+                lat.set_mask(mask)
+            lat.set_state(pops)
+            lat.step(10)
+            out.append(lat.get_state())
+            info = lat.info()
+            infos.append((info["nsolid"], tuple(info["bbox"])))
+    assert infos[0] == infos[1] and infos[0][0] == int(mask.sum())
+    assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
+
+
+def test_obstacle_list_commit_replace_and_range_check():
+    from euplectella_hpc_amd.lib import EupError
+    l, m, n = 40, 16, 12
+    announce("physics obstacle list: empty commit, replacement, range check")
+    with lattice((l, m, n), svisc=0.1, u_inflow=0.05, mode="physics", storage="f32") as lat:
+        lat.commit_obstacles()
+        assert lat.info()["nsolid"] == 0
+        lat.add_obstacles([[3, 4, 5], [3, 4, 5], [10, 1, 12]])
+        lat.commit_obstacles()
+        info = lat.info()
+        assert info["nsolid"] == 2 and tuple(info["bbox"]) == (3, 10, 1, 4, 5, 12)
+        lat.add_obstacles([[40, 16, 1]])  # a new list replaces the committed one
+        lat.commit_obstacles()
+        info = lat.info()
+        assert info["nsolid"] == 1 and tuple(info["bbox"]) == (40, 40, 16, 16, 1, 1)
+        with pytest.raises(EupError):
+            lat.add_obstacles([[41, 1, 1]])
+        with pytest.raises(EupError):
+            lat.add_obstacles([[1, 0, 1]])
+
+
 def test_momentum_exchange_closes_the_momentum_balance():
     """Discrete momentum balance of one step around the body: the momentum stored in a box after
     the step minus before = inflow through its surface - the momentum-exchange force (collision
