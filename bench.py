@@ -307,6 +307,43 @@ def stability(wl, steps, every, chain):
     return out
 
 
+def adjoint_timing(storages=("f64", "f32"), reps=3):
+    """C4 (grey lattice 128x64x64, 12 blurred spheres, tau 0.6, u_in 0.05): time of the grey forward
+    step and of one objective + gradient evaluation (mean C_D over steps 1000-2000, checkpoint
+    every 100: a 2000-step forward with checkpoints, then segment recompute and the adjoint
+    sweep back), per storage."""
+    from euplectella_hpc_amd import geometry
+    from euplectella_hpc_amd.lib import Lattice, svisc_from_tau
+
+    l, m, n, u = 128, 64, 64, 0.05
+    s = geometry.grey_spheres(l, m, n, n_spheres=12, rmin=4.0, rmax=8.0, sigma=1.5, seed=44)
+    pops = geometry.initial_state(l, m, n, u, seed=44)
+    norm = 0.5 * u ** 2 * float(s.max(axis=2).sum())
+    out = {"workload": "grey_128x64x64 (C4)", "t0": 1000, "t1": 2000, "checkpoint_every": 100}
+    for st in storages:
+        lat = Lattice(l, m, n, svisc=svisc_from_tau(0.6), u_inflow=u, mode="grey", storage=st)
+        try:
+            lat.set_grey(s)
+            lat.set_state(pops)
+            lat.step(20)
+            ms = lat.step_timed(500) / 500
+            lat.set_state(pops)
+            t = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                J, g = lat.grey_objective(1000, 2000, 100, norm)
+                t.append(time.perf_counter() - t0)
+            best = min(t)
+        finally:
+            lat.close()
+        out[st] = {"forward_ms_per_step": round(ms, 4), "forward_mlups": round(l * m * n / ms / 1e3, 1),
+                   "objective_and_gradient_s": round(best, 3),
+                   "gradient_cost_in_forward_steps": round(best * 1e3 / ms, 1), "J": J}
+        print(f"[adjoint] {st}: forward {ms:.4f} ms/step, objective + gradient {best:.3f} s "
+              f"({best * 1e3 / ms:.0f} forward steps)", file=sys.stderr, flush=True)
+    return out
+
+
 def wl_name(wl):
     return next(k for k, v in WORKLOADS.items() if v is wl)
 
@@ -331,6 +368,8 @@ def main(argv=None):
                     help="skip the higher-viscosity start-up chain (short profiling runs)")
     ap.add_argument("--no-streams", action="store_true",
                     help="skip the plain HBM copy / read / write streams")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only time the grey forward step and the adjoint gradient (C4)")
     ap.add_argument("--stability", type=int, default=0, metavar="STEPS",
                     help="only report when FP64 runs (parity and physics mode) stop being finite")
     ap.add_argument("--tau", type=float, default=0.0,
@@ -338,6 +377,9 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.tau:
         WORKLOADS[args.workload] = dict(WORKLOADS[args.workload], tau=args.tau)
+    if args.adjoint:
+        print(json.dumps(adjoint_timing()))
+        return 0
     if args.stability:
         print(json.dumps(stability(WORKLOADS[args.workload], args.stability, args.every,
                                    not args.no_chain)))
