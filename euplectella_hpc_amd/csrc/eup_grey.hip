@@ -13,10 +13,11 @@
 // Adjoint: lam^t = (d a^{t+1} / d a^t)^T lam^{t+1} + [t >= t0] w dF_x/da(a^t), and
 // dJ/ds accumulates (d a^{t+1} / ds)^T lam^{t+1} + [t >= t0] w dF_x/ds(a^t).  Per step two kernels:
 // k_adj_collide recomputes the pulled populations f(x) of a^t and applies the transposed BGK
-// Jacobian to lam^{t+1}; k_adj_stream gathers the transposed pull (all gathers, no atomics):
+// Jacobian to lam^{t+1}, and adds the gradient terms of the links the site pulls through;
+// k_adj_stream gathers the transposed pull (all gathers, no atomics):
 //   lam_p(z) = (1 - sigma_l) lin_p(z + c_p) + sigma_l lin_pbar(z)   (sigma_l of link z -- z + c_p)
 // plus the outlet-equilibrium transpose on the last x column and the objective terms, and adds the
-// site's gradient contributions from both ends of each of its links (FP64 accumulation).
+// gradient terms of the links of which the site is the source (FP64 accumulation).
 // States are checkpointed every K steps; each segment is recomputed forward and stored before its
 // backward sweep, so memory is (t1/K + K) states instead of t1.
 #include <type_traits>
@@ -28,14 +29,24 @@ namespace eup {
 
 namespace {
 
+// direction table (0-based population index; bgk3d.F90:31-49), folded into the fully unrolled
+// loops below
+constexpr int kCX[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+constexpr int kCY[19] = {-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1, 0};
+constexpr int kCZ[19] = {0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1, 0};
+constexpr int kOPP[19] = {11, 12, 9, 10, 13, 14, 15, 16, 17, 2, 3, 0, 1, 4, 5, 6, 7, 8, 18};
+
+// a state: 19 SoA arrays of l*m*n sites, population q at A + q * stride
 template <typename R> struct GArgs {
-    const R *A[19];
-    R *B[19];
+    const R *A; // the state read
+    R *B;       // the state written, or null
+    int64_t stride;
     const R *s;
     R inl[5];
     R omega;
     int l, m, n;
     int xopen;
+    __device__ __forceinline__ R a(int q, int64_t x) const { return A[q * stride + x]; }
 };
 
 struct Nbr {
@@ -58,52 +69,50 @@ struct Nbr {
     }
 };
 
-__device__ __constant__ int8_t dCX[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
-__device__ __constant__ int8_t dCY[19] = {-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1, 0};
-__device__ __constant__ int8_t dCZ[19] = {0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1, 0};
-__device__ __constant__ int8_t dOPP[19] = {11, 12, 9, 10, 13, 14, 15, 16, 17, 2, 3, 0, 1, 4, 5, 6, 7, 8, 18};
-
 template <typename R> __device__ __forceinline__ R wq(int q)
 {
-    return q == 18 ? R(1) / R(3) : ((dCX[q] * dCX[q] + dCY[q] * dCY[q] + dCZ[q] * dCZ[q]) == 1 ? R(1) / R(18) : R(1) / R(36));
+    return q == 18 ? R(1) / R(3) : ((kCX[q] * kCX[q] + kCY[q] * kCY[q] + kCZ[q] * kCZ[q]) == 1 ? R(1) / R(18) : R(1) / R(36));
 }
 
-// pulled populations f[19] of site (i,j,k) (0-based); sig[q], src[q] and kind[q]
-// (0 interior, 1 inlet, 2 outlet) describe the links, ysite[q] the source site
+// pulled populations f[19] of site (i,j,k) (0-based) and its links: sig[q] the link fraction,
+// src[q] the source value, sy[q] the source's solid fraction (0 on a ghost), kind[q] 0 interior,
+// 1 inlet, 2 outlet
 template <typename R>
-__device__ void grey_pull(const GArgs<R> &a, const Nbr &nb, int i, int j, int k, int64_t x, R *f,
-                          R *sig, R *src, int *kind, int64_t *ysite)
+__device__ __forceinline__ void grey_pull(const GArgs<R> &a, const Nbr &nb, int i, int j, int k, int64_t x, R *f,
+                                          R *sig, R *src, R *sy, int *kind)
 {
     const R sx = a.s[x];
+#pragma unroll
     for (int q = 0; q < 18; ++q) {
-        const int64_t y = nb.at(i, j, k, -dCX[q], -dCY[q], -dCZ[q]);
-        R sv, sy;
-        if (y == -1) {
-            sv = a.inl[q];
-            sy = R(0);
-            kind[q] = 1;
-            ysite[q] = -1;
-        } else if (y == -2) {
-            const int64_t z = nb.at(a.l - 1, j, k, 0, -dCY[q], -dCZ[q]);
+        const int64_t y = nb.at(i, j, k, -kCX[q], -kCY[q], -kCZ[q]);
+        R sv, syv;
+        int kd;
+        if (kCX[q] == 1 && y == -1) { // only c_x = +1 pulls from the inlet ghost
+            sv = a.inl[kCX[q] == 1 ? q : 0];
+            syv = R(0);
+            kd = 1;
+        } else if (kCX[q] == -1 && y == -2) { // only c_x = -1 pulls from the outlet ghost
+            const int64_t z = nb.at(a.l - 1, j, k, 0, -kCY[q], -kCZ[q]);
             R yv[19], o[5];
-            for (int p = 0; p < 19; ++p) yv[p] = a.A[p][z];
+#pragma unroll
+            for (int p = 0; p < 19; ++p) yv[p] = a.a(p, z);
             outlet_eq<R>(yv, o);
-            sv = o[q - 9];
-            sy = R(0);
-            kind[q] = 2;
-            ysite[q] = z;
+            sv = o[kCX[q] == -1 ? q - 9 : 0];
+            syv = R(0);
+            kd = 2;
         } else {
-            sv = a.A[q][y];
-            sy = a.s[y];
-            kind[q] = 0;
-            ysite[q] = y;
+            sv = a.a(q, y);
+            syv = a.s[y];
+            kd = 0;
         }
-        const R sg = R(1) - (R(1) - sx) * (R(1) - sy);
+        const R sg = R(1) - (R(1) - sx) * (R(1) - syv);
         sig[q] = sg;
         src[q] = sv;
-        f[q] = (R(1) - sg) * sv + sg * a.A[dOPP[q]][x];
+        sy[q] = syv;
+        kind[q] = kd;
+        f[q] = (R(1) - sg) * sv + sg * a.a(kOPP[q], x);
     }
-    f[18] = a.A[18][x];
+    f[18] = a.a(18, x);
 }
 
 template <typename R> __global__ __launch_bounds__(256) void k_grey_step(GArgs<R> a)
@@ -115,12 +124,12 @@ template <typename R> __global__ __launch_bounds__(256) void k_grey_step(GArgs<R
     const int64_t r = x / a.l;
     const int j = (int)(r % a.m), k = (int)(r / a.m);
     Nbr nb{a.l, a.m, a.n, a.xopen};
-    R f[19], sig[18], src[18];
+    R f[19], sig[18], src[18], sy[18];
     int kind[18];
-    int64_t ys[18];
-    grey_pull<R>(a, nb, i, j, k, x, f, sig, src, kind, ys);
+    grey_pull<R>(a, nb, i, j, k, x, f, sig, src, sy, kind);
     collide<R>(f, f, a.omega, R(0));
-    for (int q = 0; q < 19; ++q) a.B[q][x] = f[q];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) a.B[q * a.stride + x] = f[q];
 }
 
 __device__ __forceinline__ double wsum(double v)
@@ -139,16 +148,16 @@ template <typename R> __global__ __launch_bounds__(256) void k_grey_force(GArgs<
         const int i = (int)(x % a.l);
         const int64_t r = x / a.l;
         const int j = (int)(r % a.m), k = (int)(r / a.m);
-        R f[19], sig[18], src[18];
+        R f[19], sig[18], src[18], sy[18];
         int kind[18];
-        int64_t ys[18];
-        grey_pull<R>(a, nb, i, j, k, x, f, sig, src, kind, ys);
+        grey_pull<R>(a, nb, i, j, k, x, f, sig, src, sy, kind);
+#pragma unroll
         for (int q = 0; q < 18; ++q) {
             if (kind[q] != 0) continue;
-            const double d = (double)sig[q] * ((double)src[q] - (double)a.A[dOPP[q]][x]);
-            v[0] += dCX[q] * d;
-            v[1] += dCY[q] * d;
-            v[2] += dCZ[q] * d;
+            const double d = (double)sig[q] * ((double)src[q] - (double)a.a(kOPP[q], x));
+            v[0] += kCX[q] * d;
+            v[1] += kCY[q] * d;
+            v[2] += kCZ[q] * d;
         }
     }
     __shared__ double sh[3][4];
@@ -184,39 +193,44 @@ __global__ void k_force_acc(const double *partial, int nb, double weight, double
 }
 
 // transposed BGK Jacobian at pre-collision f applied to lam
-template <typename R> __device__ void collide_adj(const R *f, const R *lam, R omega, R *out)
+template <typename R> __device__ __forceinline__ void collide_adj(const R *f, const R *lam, R omega, R *out)
 {
     R rho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
     for (int q = 0; q < 19; ++q) {
         rho += f[q];
-        jx += dCX[q] * f[q];
-        jy += dCY[q] * f[q];
-        jz += dCZ[q] * f[q];
+        jx += kCX[q] * f[q];
+        jy += kCY[q] * f[q];
+        jz += kCZ[q] * f[q];
     }
     const R ux = jx / rho, uy = jy / rho, uz = jz / rho, usq = ux * ux + uy * uy + uz * uz;
     R A = 0, Bx = 0, By = 0, Bz = 0;
+#pragma unroll
     for (int q = 0; q < 19; ++q) {
         const R w = wq<R>(q);
-        const R cu = dCX[q] * ux + dCY[q] * uy + dCZ[q] * uz;
+        const R cu = kCX[q] * ux + kCY[q] * uy + kCZ[q] * uz;
         A += lam[q] * w * (R(1) - R(4.5) * cu * cu + R(1.5) * usq);
         const R g = lam[q] * w * (R(3) + R(9) * cu);
-        Bx += g * dCX[q] - lam[q] * w * R(3) * ux;
-        By += g * dCY[q] - lam[q] * w * R(3) * uy;
-        Bz += g * dCZ[q] - lam[q] * w * R(3) * uz;
+        Bx += g * kCX[q] - lam[q] * w * R(3) * ux;
+        By += g * kCY[q] - lam[q] * w * R(3) * uy;
+        Bz += g * kCZ[q] - lam[q] * w * R(3) * uz;
     }
+#pragma unroll
     for (int p = 0; p < 19; ++p)
-        out[p] = (R(1) - omega) * lam[p] + omega * (A + Bx * dCX[p] + By * dCY[p] + Bz * dCZ[p]);
+        out[p] = (R(1) - omega) * lam[p] + omega * (A + Bx * kCX[p] + By * kCY[p] + Bz * kCZ[p]);
 }
 
 template <typename R> struct AdjArgs {
     GArgs<R> g;
-    const R *lam_in[19];  // lam^{t+1}
-    R *lin[19];           // J_C^T lam^{t+1} at every site
-    R *lam_out[19];       // lam^t
+    const R *lam_in; // lam^{t+1} (the state layout of g)
+    R *lin;          // J_C^T lam^{t+1} at every site
+    R *lam_out;      // lam^t
     double *gs;
-    double wobj;                       // objective weight at time t (0 outside the window)
+    double wobj;     // objective weight at time t (0 outside the window)
 };
 
+// lin(x) = J_C(f(x))^T lam^{t+1}(x), and the gradient terms of the links that x pulls through
+// (x as the pulling site: d f_q(x) / d s(x) on every link, plus the objective's force term)
 template <typename R> __global__ __launch_bounds__(256) void k_adj_collide(AdjArgs<R> a)
 {
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -225,16 +239,32 @@ template <typename R> __global__ __launch_bounds__(256) void k_adj_collide(AdjAr
     const int i = (int)(x % a.g.l);
     const int64_t r = x / a.g.l;
     const int j = (int)(r % a.g.m), k = (int)(r / a.g.m);
+    const int64_t st = a.g.stride;
     Nbr nb{a.g.l, a.g.m, a.g.n, a.g.xopen};
-    R f[19], sig[18], src[18], lam[19], out[19];
+    R f[19], sig[18], src[18], sy[18], lam[19], out[19];
     int kind[18];
-    int64_t ys[18];
-    grey_pull<R>(a.g, nb, i, j, k, x, f, sig, src, kind, ys);
-    for (int q = 0; q < 19; ++q) lam[q] = a.lam_in[q][x];
+    grey_pull<R>(a.g, nb, i, j, k, x, f, sig, src, sy, kind);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) lam[q] = a.lam_in[q * st + x];
     collide_adj<R>(f, lam, a.g.omega, out);
-    for (int q = 0; q < 19; ++q) a.lin[q][x] = out[q];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) a.lin[q * st + x] = out[q];
+    const double w = a.wobj;
+    double g = 0.0;
+#pragma unroll
+    for (int q = 0; q < 18; ++q) {
+        const double L = (double)out[q];
+        const double abar = (double)a.g.a(kOPP[q], x);
+        const double syq = (double)sy[q];
+        g += L * (abar - (double)src[q]) * (1.0 - syq);
+        if (kind[q] == 0 && w != 0.0) g += w * kCX[q] * ((double)src[q] - abar) * (1.0 - syq);
+    }
+    a.gs[x] += g;
 }
 
+// lam^t(z): the transposed pull gathered from the sites z + c_p that pull from z, the
+// outlet-equilibrium transpose on the last x column and the objective's force term; and the
+// gradient terms of the links of which z is the source
 template <typename R> __global__ __launch_bounds__(256) void k_adj_stream(AdjArgs<R> a)
 {
     const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -244,69 +274,63 @@ template <typename R> __global__ __launch_bounds__(256) void k_adj_stream(AdjArg
     const int i = (int)(z % l);
     const int64_t r = z / l;
     const int j = (int)(r % m), k = (int)(r / m);
+    const int64_t st = a.g.stride;
     Nbr nb{l, m, n, a.g.xopen};
     const double w = a.wobj;
     const R sz = a.g.s[z];
     double g = 0.0;
-    // (A) z as the pulling site x
-    {
-        R f[19], sig[18], src[18];
-        int kind[18];
-        int64_t ys[18];
-        grey_pull<R>(a.g, nb, i, j, k, z, f, sig, src, kind, ys);
-        for (int q = 0; q < 18; ++q) {
-            const double L = (double)a.lin[q][z];
-            const double abar = (double)a.g.A[dOPP[q]][z];
-            const double sy = kind[q] == 0 ? (double)a.g.s[ys[q]] : 0.0;
-            g += L * (abar - (double)src[q]) * (1.0 - sy);
-            if (kind[q] == 0 && w != 0.0) g += w * dCX[q] * ((double)src[q] - abar) * (1.0 - sy);
-        }
-    }
+#pragma unroll
     for (int p = 0; p < 18; ++p) {
-        const int pb = dOPP[p];
-        const int64_t zp = nb.at(i, j, k, dCX[p], dCY[p], dCZ[p]); // the site pulling p from z
+        const int pb = kOPP[p];
+        const int64_t zp = nb.at(i, j, k, kCX[p], kCY[p], kCZ[p]); // the site pulling p from z
         double lam;
         if (zp >= 0) {
             const R s2 = a.g.s[zp];
             const double sl = 1.0 - (1.0 - (double)sz) * (1.0 - (double)s2);
-            lam = (1.0 - sl) * (double)a.lin[p][zp] + sl * (double)a.lin[pb][z];
-            if (w != 0.0) lam += 2.0 * w * sl * dCX[p];
-            // (B) z as the source y of the pair (zp, p)
-            const double d = (double)a.g.A[pb][zp] - (double)a.g.A[p][z];
-            g += (double)a.lin[p][zp] * d * (1.0 - (double)s2);
-            if (w != 0.0) g -= w * dCX[p] * d * (1.0 - (double)s2);
+            const double Lp = (double)a.lin[p * st + zp];
+            lam = (1.0 - sl) * Lp + sl * (double)a.lin[pb * st + z];
+            if (w != 0.0) lam += 2.0 * w * sl * kCX[p];
+            const double d = (double)a.g.a(pb, zp) - (double)a.g.a(p, z);
+            g += Lp * d * (1.0 - (double)s2);
+            if (w != 0.0) g -= w * kCX[p] * d * (1.0 - (double)s2);
         } else {
-            lam = (double)sz * (double)a.lin[pb][z]; // ghost link: sigma = s(z)
+            lam = (double)sz * (double)a.lin[pb * st + z]; // ghost link: sigma = s(z)
         }
-        a.lam_out[p][z] = (R)lam;
+        a.lam_out[p * st + z] = (R)lam;
     }
-    a.lam_out[18][z] = a.lin[18][z];
+    a.lam_out[18 * st + z] = a.lin[18 * st + z];
     // outlet equilibrium transpose: z on the last x column feeds the ghost values read by the
     // sites x = (l-1, j + cy, k + cz) for q = 10..14 (0-based 9..13)
     if (a.g.xopen && i == l - 1) {
         R y[19];
-        for (int p = 0; p < 19; ++p) y[p] = a.g.A[p][z];
+#pragma unroll
+        for (int p = 0; p < 19; ++p) y[p] = a.g.a(p, z);
         double Jx = 0, Jy = 0, Jz = 0;
+#pragma unroll
         for (int p = 0; p < 19; ++p) {
-            Jx += dCX[p] * (double)y[p];
-            Jy += dCY[p] * (double)y[p];
-            Jz += dCZ[p] * (double)y[p];
+            Jx += kCX[p] * (double)y[p];
+            Jy += kCY[p] * (double)y[p];
+            Jz += kCZ[p] * (double)y[p];
         }
         double acc[19];
+#pragma unroll
         for (int p = 0; p < 19; ++p) acc[p] = 0.0;
+#pragma unroll
         for (int q = 9; q < 14; ++q) {
-            const int64_t x = nb.at(l - 1, j, k, 0, dCY[q], dCZ[q]);
+            const int64_t x = nb.at(l - 1, j, k, 0, kCY[q], kCZ[q]);
             const double sg = (double)a.g.s[x]; // sigma of the ghost link
-            const double L = (1.0 - sg) * (double)a.lin[q][x];
+            const double L = (1.0 - sg) * (double)a.lin[q * st + x];
             const double wv = (double)wq<double>(q);
-            const double cJ = dCX[q] * Jx + dCY[q] * Jy + dCZ[q] * Jz;
+            const double cJ = kCX[q] * Jx + kCY[q] * Jy + kCZ[q] * Jz;
+#pragma unroll
             for (int p = 0; p < 19; ++p) {
-                const double cc = dCX[q] * dCX[p] + dCY[q] * dCY[p] + dCZ[q] * dCZ[p];
-                const double Jc = Jx * dCX[p] + Jy * dCY[p] + Jz * dCZ[p];
+                const double cc = kCX[q] * kCX[p] + kCY[q] * kCY[p] + kCZ[q] * kCZ[p];
+                const double Jc = Jx * kCX[p] + Jy * kCY[p] + Jz * kCZ[p];
                 acc[p] += L * wv * (3.0 * cc + 9.0 * cJ * cc - 3.0 * Jc);
             }
         }
-        for (int p = 0; p < 19; ++p) a.lam_out[p][z] = (R)((double)a.lam_out[p][z] + acc[p]);
+#pragma unroll
+        for (int p = 0; p < 19; ++p) a.lam_out[p * st + z] = (R)((double)a.lam_out[p * st + z] + acc[p]);
     }
     a.gs[z] += g;
 }
@@ -321,41 +345,41 @@ template <typename R> __global__ __launch_bounds__(256) void k_force_adj_seed(Ad
     const int i = (int)(z % l);
     const int64_t r = z / l;
     const int j = (int)(r % m), k = (int)(r / m);
+    const int64_t st = a.g.stride;
     Nbr nb{l, m, n, a.g.xopen};
     const double w = a.wobj;
     const double sz = (double)a.g.s[z];
     double g = 0.0;
+#pragma unroll
     for (int q = 0; q < 18; ++q) { // z as x: pair (z, q) with interior source y = z - c_q
-        const int64_t y = nb.at(i, j, k, -dCX[q], -dCY[q], -dCZ[q]);
+        const int64_t y = nb.at(i, j, k, -kCX[q], -kCY[q], -kCZ[q]);
         if (y < 0) continue;
-        const double d = (double)a.g.A[q][y] - (double)a.g.A[dOPP[q]][z];
-        g += w * dCX[q] * d * (1.0 - (double)a.g.s[y]);
+        const double d = (double)a.g.a(q, y) - (double)a.g.a(kOPP[q], z);
+        g += w * kCX[q] * d * (1.0 - (double)a.g.s[y]);
     }
+#pragma unroll
     for (int p = 0; p < 18; ++p) {
-        const int64_t zp = nb.at(i, j, k, dCX[p], dCY[p], dCZ[p]);
+        const int64_t zp = nb.at(i, j, k, kCX[p], kCY[p], kCZ[p]);
         double lam = 0.0;
         if (zp >= 0) {
             const double s2 = (double)a.g.s[zp];
             const double sl = 1.0 - (1.0 - sz) * (1.0 - s2);
-            lam = 2.0 * w * sl * dCX[p];
-            const double d = (double)a.g.A[p][z] - (double)a.g.A[dOPP[p]][zp];
-            g += w * dCX[p] * d * (1.0 - s2);
+            lam = 2.0 * w * sl * kCX[p];
+            const double d = (double)a.g.a(p, z) - (double)a.g.a(kOPP[p], zp);
+            g += w * kCX[p] * d * (1.0 - s2);
         }
-        a.lam_out[p][z] = (R)lam;
+        a.lam_out[p * st + z] = (R)lam;
     }
-    a.lam_out[18][z] = R(0);
+    a.lam_out[18 * st + z] = R(0);
     a.gs[z] += g;
 }
 
 template <typename R> GArgs<R> gargs(Handle *h, const void *from, void *to)
 {
     GArgs<R> a;
-    const R *src = static_cast<const R *>(from);
-    R *dst = static_cast<R *>(to);
-    for (int q = 0; q < 19; ++q) {
-        a.A[q] = src + (int64_t)q * h->stride;
-        a.B[q] = dst ? dst + (int64_t)q * h->stride : nullptr;
-    }
+    a.A = static_cast<const R *>(from);
+    a.B = static_cast<R *>(to);
+    a.stride = h->stride;
     a.s = static_cast<const R *>(h->grey);
     R inl[5];
     inlet_eq<R>((R)h->cfg.u_inflow, inl);
@@ -468,11 +492,9 @@ int objective_t(Handle *h, int t0, int t1, int K, double norm, double *J, void *
         auto adj = [&](const void *state, int li, int lo, double w) {
             AdjArgs<R> q;
             q.g = gargs<R>(h, state, nullptr);
-            for (int p = 0; p < 19; ++p) {
-                q.lam_in[p] = static_cast<const R *>(lamb[li]) + (int64_t)p * h->stride;
-                q.lin[p] = static_cast<R *>(lin) + (int64_t)p * h->stride;
-                q.lam_out[p] = static_cast<R *>(lamb[lo]) + (int64_t)p * h->stride;
-            }
+            q.lam_in = static_cast<const R *>(lamb[li]);
+            q.lin = static_cast<R *>(lin);
+            q.lam_out = static_cast<R *>(lamb[lo]);
             q.gs = gs;
             q.wobj = w;
             return q;
