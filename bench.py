@@ -91,12 +91,12 @@ def parse_storage(spec):
     return (spec[:-1], True) if spec in ("f16s", "bf16s") else (spec, False)
 
 
-def build_lattice(wl, storage, mask, chain=True):
+def build_lattice(wl, storage, mask, chain=True, mode="physics"):
     from euplectella_hpc_amd.lib import Lattice, svisc_from_tau
 
     l, m, n = wl["l"], wl["m"], wl["n"]
     st, shifted = parse_storage(storage)
-    lat = Lattice(l, m, n, svisc=svisc_from_tau(wl["tau"]), u_inflow=wl["u_in"], mode="physics",
+    lat = Lattice(l, m, n, svisc=svisc_from_tau(wl["tau"]), u_inflow=wl["u_in"], mode=mode,
                   storage=st, shifted=shifted, x_open=wl.get("x_open", True))
     lat.set_mask(mask)
     # u = (u_in, +-1e-4 seeded noise, 0) at every site, generated on the device
@@ -113,8 +113,8 @@ def build_lattice(wl, storage, mask, chain=True):
     return lat
 
 
-def run_one(wl, storage, mask, steps, warmup, chain=True):
-    lat = build_lattice(wl, storage, mask, chain)
+def run_one(wl, storage, mask, steps, warmup, chain=True, mode="physics"):
+    lat = build_lattice(wl, storage, mask, chain, mode)
     try:
         info = lat.info()
         lat.step(warmup)
@@ -126,7 +126,8 @@ def run_one(wl, storage, mask, steps, warmup, chain=True):
         _torch_sync()
         wall = time.perf_counter() - t0
         sites = wl["l"] * wl["m"] * wl["n"]
-        fluid = sites - info["nsolid"]
+        # physics mode neither reads nor writes solid sites; parity mode updates every site
+        fluid = sites - info["nsolid"] if mode == "physics" else sites
         kernel_s = gpu_ms / 1e3 / steps
         bps = 2 * 19 * storage_bytes(storage)
         achieved = fluid * bps / kernel_s / 1e9
@@ -355,6 +356,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="sponge_512x256x256", choices=sorted(WORKLOADS))
     ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16", "bf16", "f16s", "bf16s"])
+    ap.add_argument("--mode", default="physics", choices=["physics", "parity"],
+                    help="parity: the reference's serial FP64 step bit for bit (storage f64, no secondary)")
     ap.add_argument("--secondary", default="f16", help="second storage to report ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -405,7 +408,9 @@ def main(argv=None):
     mask = workload_mask(wl)
     print(f"[bench] {args.workload}: geometry ready, {int(mask.sum())} solid sites", file=sys.stderr,
           flush=True)
-    prim = run_one(wl, args.storage, mask, args.steps, args.warmup, not args.no_chain)
+    if args.mode == "parity":
+        args.storage, args.secondary = "f64", ""
+    prim = run_one(wl, args.storage, mask, args.steps, args.warmup, not args.no_chain, args.mode)
     if not prim["finite"]:
         print(f"bench.py: the {args.storage} state is not finite after the timed steps",
               file=sys.stderr)
@@ -418,7 +423,8 @@ def main(argv=None):
         print(f"[bench] {args.secondary}: {sec['mlups']:.0f} MLUPS, kernel {sec['kernel_ms']:.3f} ms",
               file=sys.stderr, flush=True)
     streams = None if args.no_streams else hbm_streams()
-    traffic, traffic_src = profiled_traffic(args.workload, args.storage)
+    traffic, traffic_src = (profiled_traffic(args.workload, args.storage) if args.mode == "physics"
+                            else (None, None))
     cpu = None if args.no_cpu else cpu_baseline(wl)
     out = {
         "metric": "MLUPS + % HBM peak, FP32 vs FP16-storage at |dC_D|<=1e-3 vs FP64",
@@ -436,7 +442,7 @@ def main(argv=None):
         "data": "synthetic (seeded sponge geometry, u = inlet + seeded noise)",
         "config": {"workload": args.workload, "lattice": [wl["l"], wl["m"], wl["n"]],
                    "tau": wl["tau"], "u_in": wl["u_in"], "storage": args.storage,
-                   "mode": "physics", "parallelism": "single GPU",
+                   "mode": args.mode, "parallelism": "single GPU",
                    "x_faces": "open (inlet / outlet)" if wl.get("x_open", True) else
                    f"periodic (stand-in for the halo of x-slab {wl['crop_x']} of {wl.get('parent')})"},
         "mflups_fluid": round(prim["mflups_fluid"], 1),
@@ -444,7 +450,9 @@ def main(argv=None):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(prim["frac"], 4),
                      "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes": int(prim["fluid_sites"] * prim["bytes_per_site"]),
-                     "kernel": "k_fused", "kernel_ms": round(prim["kernel_ms"], 4),
+                     "kernel": "k_fused" if args.mode == "physics" else
+                     "k_bb_gather + k_bb_scatter + k_ghost + k_stream_collide",
+                     "kernel_ms": round(prim["kernel_ms"], 4),
                      "bytes_per_site": prim["bytes_per_site"],
                      "hbm_streams_gbs": streams,
                      "frac_of_copy": (round(prim["achieved_gbs"] / streams["copy"], 4)

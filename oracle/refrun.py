@@ -127,7 +127,7 @@ def time_reference(l, m, n, svisc, u0, steps, mask, *, threads=None, variant="om
     try:
         big = steps + 10
         fortio.write_deck(os.path.join(wd, "bgk.input"), svisc=float(svisc), u0=float(u0),
-                          itfin=int(steps), ivtim=int(ivtim if ivtim else big), isignal=big, itsave=big, icheck=big,
+                          itfin=int(steps), ivtim=big, isignal=big, itsave=big, icheck=big,
                           irestart=0, init_v=0, lx=l, ly=m, lz=n, proc_x=1, proc_y=1, proc_z=1,
                           tstep=1, flag1=1, flag2=2, flag3=0)
         fortio.write_obs(os.path.join(wd, fortio.obs_file_name()), mask)

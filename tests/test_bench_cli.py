@@ -3,6 +3,8 @@ import os
 import subprocess
 import sys
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -24,3 +26,14 @@ def test_workloads_define_the_configured_lattices():
     assert (c5["l"], c5["m"], c5["n"], c5["radius"], c5["cell"]) == (1536, 768, 768, 144, 48)
     c1 = bench.WORKLOADS["cylinder_128x64x64"]
     assert (c1["radius"], c1["centre"], c1["tau"], c1["u_in"]) == (8, (32, 32), 0.6, 0.05)
+
+
+def test_cpu_baseline_leg_runs_the_reference():
+    """The CPU-baseline leg (the reference's OpenMP build, bounded sample) on a tiny lattice."""
+    from oracle import refrun
+    from euplectella_hpc_amd import geometry
+    if refrun.ref_binary("omp") is None:
+        pytest.skip("the reference build is not present")
+    mask = geometry.cylinder(24, 12, 8, 2, (6, 6.5))
+    r = refrun.time_reference(24, 12, 8, (2 * 0.6 - 1) / 6, 0.05, 5, mask, threads=2)
+    assert r["mlups"] > 0 and r["threads"] == 2
