@@ -16,7 +16,8 @@
 //      are written into both A and B, because the next step's bounce-back reads the ghosts left
 //      by this step's fill (the one-step-stale seam of the serial path).
 //   3. k_stream_collide: movef.F90:43-68 fused with col.F90:96-235 (pull from A, collide, write
-//      the interior of B); the FP64 collision is the reference's expression order.
+//      the interior of B); the FP64 collision is the reference's expression order.  One wavefront
+//      per 64-site row segment, so the nine source rows are wave-uniform pointers.
 #include "eup_internal.h"
 
 namespace eup {
@@ -26,6 +27,9 @@ namespace {
 struct PArgs {
     const double *A[19];
     double *B[19];
+    const double *A0; // population 0 of A; population q at A0 + q * stride
+    double *B0;
+    int64_t stride;
     int l, m, n;
     double omega, fgrad, u_in;
 };
@@ -162,38 +166,45 @@ __global__ void k_ghost(PArgs a)
     }
 }
 
+// pull of population Q into x: the source row (j - c_y, k - c_z) is wave-uniform, so its pointer
+// is formed in scalar registers (base pointer + population and row offsets); the lane adds the
+// zero-extended x offset i - c_x >= 0 (ghost column 0 included)
 template <int Q>
-__device__ __forceinline__ void pull_one(const PArgs &a, int64_t s, int64_t lp, int64_t lpmp,
+__device__ __forceinline__ void pull_one(const PArgs &a, int64_t row, int64_t lp, int64_t lpmp, int i,
                                          double *x)
 {
     constexpr int cx = Dir<Q>::cx, cy = Dir<Q>::cy, cz = Dir<Q>::cz;
-    x[Q] = a.A[Q][s - cx - cy * lp - cz * lpmp];
+    const double *p = a.A0 + ((int64_t)Q * a.stride + row - cy * lp - cz * lpmp);
+    x[Q] = p[(uint64_t)(uint32_t)(i - cx)];
 }
 
 template <int... Qs>
-__device__ __forceinline__ void pull_all(QSeq<Qs...>, const PArgs &a, int64_t s, int64_t lp,
-                                         int64_t lpmp, double *x)
+__device__ __forceinline__ void pull_all(QSeq<Qs...>, const PArgs &a, int64_t row, int64_t lp,
+                                         int64_t lpmp, int i, double *x)
 {
-    (pull_one<Qs>(a, s, lp, lpmp, x), ...);
+    (pull_one<Qs>(a, row, lp, lpmp, i, x), ...);
 }
 
-__global__ __launch_bounds__(256) void k_stream_collide(PArgs a)
+// one wavefront per 64-site segment of an interior row (tpr segments per row): j and k are
+// wave-uniform, i = 1 + 64 * segment + lane
+__global__ __launch_bounds__(256) void k_stream_collide(PArgs a, int tpr, uint32_t segments)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t N = (int64_t)a.l * a.m * a.n;
-    if (t >= N) return;
-    const int i = 1 + (int)(t % a.l);
-    const int64_t r = t / a.l;
-    const int j = 1 + (int)(r % a.m);
-    const int k = 1 + (int)(r / a.m);
+    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6));
+    if (w >= segments) return;
+    const uint32_t rowi = w / (uint32_t)tpr;
+    const int seg = (int)(w - rowi * (uint32_t)tpr);
+    const int j = 1 + (int)(rowi % (uint32_t)a.m);
+    const int k = 1 + (int)(rowi / (uint32_t)a.m);
+    const int i = 1 + seg * 64 + (int)(threadIdx.x & 63);
+    if (i > a.l) return;
     const int64_t lp = a.l + 2, lpmp = lp * (a.m + 2);
-    const int64_t s = i + lp * j + lpmp * k;
+    const int64_t row = lp * j + lpmp * k;
     double x[19];
-    pull_all(Q18{}, a, s, lp, lpmp, x);
-    x[18] = a.A[18][s];
+    pull_all(Q18{}, a, row, lp, lpmp, i, x);
+    x[18] = (a.A0 + (18 * a.stride + row))[(uint64_t)(uint32_t)i];
     collide<double>(x, x, a.omega, a.fgrad);
 #pragma unroll
-    for (int q = 0; q < 19; ++q) a.B[q][s] = x[q];
+    for (int q = 0; q < 19; ++q) (a.B0 + ((int64_t)q * a.stride + row))[(uint64_t)(uint32_t)i] = x[q];
 }
 
 PArgs make_args(Handle *h, int from)
@@ -205,6 +216,9 @@ PArgs make_args(Handle *h, int from)
         a.A[q] = src + (int64_t)q * h->stride;
         a.B[q] = dst + (int64_t)q * h->stride;
     }
+    a.A0 = src;
+    a.B0 = dst;
+    a.stride = h->stride;
     a.l = h->g.l;
     a.m = h->g.m;
     a.n = h->g.n;
@@ -272,7 +286,9 @@ int parity_step(Handle *h)
     }
     const int64_t gwork = 2 * (int64_t)(g.m + 2) * (g.n + 2) + (int64_t)g.l * g.m + (int64_t)g.l * g.n;
     hipLaunchKernelGGL(k_ghost, dim3(nblocks(gwork, 256)), dim3(256), 0, h->stream, a);
-    hipLaunchKernelGGL(k_stream_collide, dim3(nblocks(g.sites, 256)), dim3(256), 0, h->stream, a);
+    const int tpr = (g.l + 63) / 64;
+    const uint32_t segments = (uint32_t)((int64_t)tpr * g.m * g.n);
+    hipLaunchKernelGGL(k_stream_collide, dim3((segments + 3) / 4), dim3(256), 0, h->stream, a, tpr, segments);
     if (hip_check(h, hipGetLastError(), "parity step launch")) return EUP_EHIP;
     h->cur ^= 1;
     return 0;
