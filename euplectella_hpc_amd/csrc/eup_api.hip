@@ -201,7 +201,20 @@ int eup_create(const eup_config *cfg, eup_handle **out)
     g.tiles_per_row = (g.l + kTile - 1) / kTile;
     g.tiles = (int64_t)g.tiles_per_row * g.m * g.n;
     h->elem = storage_bytes(cfg->storage);
-    const int64_t per = cfg->mode == EUP_MODE_PARITY ? g.ghosted : g.sites;
+    // parity mode: the reference's ghosted arrays with the row pitch padded to 16 elements and an
+    // offset of 15, so that every interior row starts on a 128-byte line (the population stride is a
+    // multiple of 256 elements); other modes: dense l*m*n per population
+    if (cfg->mode == EUP_MODE_PARITY) {
+        h->ppitch = (g.l + 2 + 15) / 16 * 16;
+        h->poff = 15;
+    }
+    const int64_t per = cfg->mode == EUP_MODE_PARITY
+                            ? h->poff + (int64_t)h->ppitch * (g.m + 2) * (g.n + 2)
+                            : g.sites;
+    if (cfg->mode == EUP_MODE_PARITY && per >= 2147483647LL) {
+        delete h;
+        return fail(nullptr, EUP_EINVAL, "eup_create: parity-mode lattice too large (padded ghosted size >= 2^31)");
+    }
     h->stride = (per + 255) / 256 * 256;
     h->tiled = cfg->mode == EUP_MODE_PHYSICS;
     int rc = 0;

@@ -20,11 +20,11 @@ constexpr int kRedBlocks = 2048;
 
 // population q of interior site (i, j, k) (1-based) as double
 struct ParityAcc {
-    const double *p[19];
-    int l, m, n;
+    const double *p[19]; // population q at its offset base
+    int l, m, n, pitch;
     __device__ __forceinline__ double operator()(int q, int i, int j, int k) const
     {
-        return p[q][(int64_t)i + (int64_t)(l + 2) * ((int64_t)j + (int64_t)(m + 2) * k)];
+        return p[q][(int64_t)i + (int64_t)pitch * ((int64_t)j + (int64_t)(m + 2) * k)];
     }
 };
 
@@ -320,10 +320,11 @@ ParityAcc parity_acc(Handle *h)
 {
     ParityAcc a;
     const double *base = static_cast<const double *>(h->buf[h->cur]);
-    for (int q = 0; q < 19; ++q) a.p[q] = base + (int64_t)q * h->stride;
+    for (int q = 0; q < 19; ++q) a.p[q] = base + (int64_t)q * h->stride + h->poff;
     a.l = h->g.l;
     a.m = h->g.m;
     a.n = h->g.n;
+    a.pitch = h->ppitch;
     return a;
 }
 

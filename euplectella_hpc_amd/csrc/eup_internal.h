@@ -35,6 +35,8 @@ struct Handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t elem = 8;              // bytes per stored population value
     int64_t stride = 0;           // elements between populations in an SoA state buffer
+    int ppitch = 0;               // parity mode: row pitch of the ghosted layout (elements)
+    int64_t poff = 0;             // parity mode: element offset of site (0, 0, 0) in a population
     bool tiled = false;           // physics mode: tile-major layout (PopIndex)
     void *alloc[2] = {nullptr, nullptr}; // device allocations of the two state buffers
     void *buf[2] = {nullptr, nullptr}; // ping-pong population buffers (past the leading guard)
@@ -59,6 +61,7 @@ struct Handle {
     void *outlet[2] = {nullptr, nullptr}; // outlet ghost plane, 5 x m x n (compute type)
     bool outlet_dirty = true;
     size_t fused_lds = (size_t)-1; // dynamic LDS per block of the fused kernel (occupancy cap)
+    size_t parity_lds = (size_t)-1; // the same for the parity stream-collide kernel
     // diagnostics scratch
     double *diag = nullptr;       // device scratch for reductions
     int64_t diag_len = 0;
@@ -70,6 +73,7 @@ struct Handle {
 };
 
 int fail(Handle *h, int code, const std::string &msg);
+size_t occupancy_lds_bytes(const Handle *h, int cap);
 int hip_check(Handle *h, hipError_t e, const char *what);
 
 // parity mode (eup_parity.hip)
@@ -102,15 +106,17 @@ int diag_vtk_velocity(Handle *h, int plane, int index, float *out);
 int diag_vtk_vorticity(Handle *h, int k0, float *out);
 
 // Element index of population q of site (i, j, k) (0-based) in a dense state buffer.
-//  * SoA (grey mode): q * stride + i + l * (j + m * k).
+//  * SoA: q * stride + off + i + pitch * (j + m * k); grey mode: pitch = l, off = 0; parity
+//    mode: ghosted coordinates (0..l+1, 0..m+1, 0..n+1) with m = m + 2, pitch = the padded row
+//    pitch and off the offset that puts interior site i = 1 on a 128-byte line.
 //  * tile-major (physics mode): the lattice is cut into tiles of 64 consecutive sites along x
 //    (tpr = ceil(l / 64) per row, the last one padded); tile t = i / 64 + tpr * (j + m * k) holds
 //    its populations as 19 runs of 64 values, so population q of the site sits at
 //    (t * 19 + q) * 64 + i % 64.  One guard tile precedes tile 0 and one follows the last, so that
 //    a pull shifted by one site in x from the first or last tile stays inside the allocation.
 struct PopIndex {
-    int64_t stride;
-    int l, m, tpr, tiled;
+    int64_t stride, off;
+    int pitch, m, tpr, tiled;
     __host__ __device__ __forceinline__ int64_t operator()(int q, int i, int j, int k) const
     {
         const int64_t row = (int64_t)j + (int64_t)m * k;
@@ -118,16 +124,18 @@ struct PopIndex {
             const int64_t t = (int64_t)(i >> 6) + (int64_t)tpr * row;
             return ((t * 19 + q) << 6) + (i & 63);
         }
-        return (int64_t)q * stride + i + (int64_t)l * row;
+        return (int64_t)q * stride + off + i + (int64_t)pitch * row;
     }
 };
 
 inline PopIndex pop_index(const Handle *h)
 {
     PopIndex p;
+    const bool parity = h->cfg.mode == EUP_MODE_PARITY;
     p.stride = h->stride;
-    p.l = h->g.l;
-    p.m = h->g.m;
+    p.off = parity ? h->poff : 0;
+    p.pitch = parity ? h->ppitch : h->g.l;
+    p.m = parity ? h->g.m + 2 : h->g.m;
     p.tpr = h->g.tiles_per_row;
     p.tiled = h->tiled ? 1 : 0;
     return p;
@@ -148,9 +156,10 @@ inline DevRef dev_ref(const Handle *h)
     return d;
 }
 
-inline int64_t ghosted_index(const Geometry &g, int i, int j, int k)
+// parity layout: element of ghosted site (i, j, k) in a population, relative to the offset base
+inline int64_t parity_index(const Handle *h, int i, int j, int k)
 {
-    return (int64_t)i + (int64_t)(g.l + 2) * ((int64_t)j + (int64_t)(g.m + 2) * (int64_t)k);
+    return (int64_t)i + (int64_t)h->ppitch * ((int64_t)j + (int64_t)(h->g.m + 2) * (int64_t)k);
 }
 
 } // namespace eup

@@ -18,6 +18,8 @@
 //   3. k_stream_collide: movef.F90:43-68 fused with col.F90:96-235 (pull from A, collide, write
 //      the interior of B); the FP64 collision is the reference's expression order.  One wavefront
 //      per 64-site row segment, so the nine source rows are wave-uniform pointers.
+#include <stdlib.h>
+
 #include "eup_internal.h"
 
 namespace eup {
@@ -30,6 +32,7 @@ struct PArgs {
     const double *A0; // population 0 of A; population q at A0 + q * stride
     double *B0;
     int64_t stride;
+    int pitch;        // row pitch (elements) of the ghosted layout
     int l, m, n;
     double omega, fgrad, u_in;
 };
@@ -50,7 +53,7 @@ __device__ __forceinline__ void bb_one(const PArgs &a, const uint8_t *solid, int
                                        int64_t s, double *out)
 {
     constexpr int cx = Dir<Q>::cx, cy = Dir<Q>::cy, cz = Dir<Q>::cz, op = Dir<Q>::opp;
-    const int lp = a.l + 2, mp = a.m + 2;
+    const int lp = a.pitch, mp = a.m + 2;
     const int yi = i + cx, yj = j + cy, yk = k + cz;
     bool ysolid = false;
     if (yi >= 1 && yi <= a.l && yj >= 1 && yj <= a.m && yk >= 1 && yk <= a.n)
@@ -77,7 +80,7 @@ __global__ void k_bb_gather(PArgs a, const int32_t *list, int64_t nsolid, const 
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsolid) return;
-    const int lp = a.l + 2, mp = a.m + 2;
+    const int lp = a.pitch, mp = a.m + 2;
     const int64_t s = list[t];
     const int i = (int)(s % lp);
     const int j = (int)((s / lp) % mp);
@@ -107,7 +110,7 @@ __device__ __forceinline__ void put2(const PArgs &a, int q, int64_t s, double v)
 // l*n y-face rows; P = (m+2)(n+2)
 __global__ void k_ghost(PArgs a)
 {
-    const int l = a.l, m = a.m, n = a.n, lp = l + 2, mp = m + 2;
+    const int l = a.l, m = a.m, n = a.n, lp = a.pitch, mp = m + 2;
     const int64_t P = (int64_t)(m + 2) * (n + 2);
     const int64_t Z = (int64_t)l * m, Y = (int64_t)l * n;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -197,7 +200,7 @@ __global__ __launch_bounds__(256) void k_stream_collide(PArgs a, int tpr, uint32
     const int k = 1 + (int)(rowi / (uint32_t)a.m);
     const int i = 1 + seg * 64 + (int)(threadIdx.x & 63);
     if (i > a.l) return;
-    const int64_t lp = a.l + 2, lpmp = lp * (a.m + 2);
+    const int64_t lp = a.pitch, lpmp = lp * (a.m + 2);
     const int64_t row = lp * j + lpmp * k;
     double x[19];
     pull_all(Q18{}, a, row, lp, lpmp, i, x);
@@ -207,18 +210,44 @@ __global__ __launch_bounds__(256) void k_stream_collide(PArgs a, int tpr, uint32
     for (int q = 0; q < 19; ++q) (a.B0 + ((int64_t)q * a.stride + row))[(uint64_t)(uint32_t)i] = x[q];
 }
 
+// one population between the reference's ghosted array (row pitch l + 2; the staging buffer) and
+// the padded layout (row pitch `pitch`), ghosted site t = i + (l+2) (j + (m+2) k)
+__global__ void k_parity_put(const double *__restrict__ stg, double *__restrict__ d0, double *__restrict__ d1,
+                             int l, int m, int n, int pitch)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)(l + 2) * (m + 2) * (n + 2)) return;
+    const int i = (int)(t % (l + 2));
+    const int64_t r = t / (l + 2);
+    const int64_t e = i + (int64_t)pitch * r;
+    d0[e] = stg[t];
+    d1[e] = stg[t];
+}
+
+__global__ void k_parity_take(const double *__restrict__ src, double *__restrict__ stg, int l, int m, int n,
+                              int pitch)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)(l + 2) * (m + 2) * (n + 2)) return;
+    const int i = (int)(t % (l + 2));
+    const int64_t r = t / (l + 2);
+    stg[t] = src[i + (int64_t)pitch * r];
+}
+
 PArgs make_args(Handle *h, int from)
 {
     PArgs a;
     double *src = static_cast<double *>(h->buf[from]);
     double *dst = static_cast<double *>(h->buf[from ^ 1]);
+    // every population pointer sits on its offset base (ghosted site (0, 0, 0))
     for (int q = 0; q < 19; ++q) {
-        a.A[q] = src + (int64_t)q * h->stride;
-        a.B[q] = dst + (int64_t)q * h->stride;
+        a.A[q] = src + (int64_t)q * h->stride + h->poff;
+        a.B[q] = dst + (int64_t)q * h->stride + h->poff;
     }
-    a.A0 = src;
-    a.B0 = dst;
+    a.A0 = src + h->poff;
+    a.B0 = dst + h->poff;
     a.stride = h->stride;
+    a.pitch = h->ppitch;
     a.l = h->g.l;
     a.m = h->g.m;
     a.n = h->g.n;
@@ -234,7 +263,7 @@ inline unsigned nblocks(int64_t work, int bs) { return (unsigned)((work + bs - 1
 
 int parity_prepare_mask(Handle *h)
 {
-    // solid list in the reference's bbox loop order (k, j, i), as ghosted offsets
+    // solid list in the reference's bbox loop order (k, j, i), as offsets in the padded layout
     const Geometry &g = h->g;
     uint8_t *sol_h = nullptr;
     if (hip_check(h, hipHostMalloc((void **)&sol_h, g.sites), "hipHostMalloc")) return EUP_EHIP;
@@ -254,7 +283,7 @@ int parity_prepare_mask(Handle *h)
         for (int j = 1; j <= g.m; ++j)
             for (int i = 1; i <= g.l; ++i) {
                 int64_t o = (int64_t)(i - 1) + (int64_t)g.l * ((int64_t)(j - 1) + (int64_t)g.m * (k - 1));
-                if (sol_h[o] != 0) list_h[c++] = (int32_t)ghosted_index(g, i, j, k);
+                if (sol_h[o] != 0) list_h[c++] = (int32_t)parity_index(h, i, j, k);
             }
     hipHostFree(sol_h);
     hipFree(h->solid_list);
@@ -288,7 +317,16 @@ int parity_step(Handle *h)
     hipLaunchKernelGGL(k_ghost, dim3(nblocks(gwork, 256)), dim3(256), 0, h->stream, a);
     const int tpr = (g.l + 63) / 64;
     const uint32_t segments = (uint32_t)((int64_t)tpr * g.m * g.n);
-    hipLaunchKernelGGL(k_stream_collide, dim3((segments + 3) / 4), dim3(256), 0, h->stream, a, tpr, segments);
+    if (h->parity_lds == (size_t)-1) {
+        // occupancy cap of the stream-collide (as for the fused FP64 kernel): 3 blocks per CU run
+        // about 4 % faster on MI355X than the register-limited occupancy;
+        // EUP_PARITY_BLOCKS_PER_CU overrides (0 = no cap)
+        int cap = 3;
+        if (const char *e = getenv("EUP_PARITY_BLOCKS_PER_CU")) cap = atoi(e);
+        h->parity_lds = occupancy_lds_bytes(h, cap);
+    }
+    hipLaunchKernelGGL(k_stream_collide, dim3((segments + 3) / 4), dim3(256), h->parity_lds, h->stream, a, tpr,
+                       segments);
     if (hip_check(h, hipGetLastError(), "parity step launch")) return EUP_EHIP;
     h->cur ^= 1;
     return 0;
@@ -296,22 +334,21 @@ int parity_step(Handle *h)
 
 int parity_set_pop(Handle *h, int q)
 {
-    for (int b = 0; b < 2; ++b) {
-        double *dst = static_cast<double *>(h->buf[b]) + (int64_t)(q - 1) * h->stride;
-        if (hip_check(h, hipMemcpyAsync(dst, h->staging, sizeof(double) * h->g.ghosted,
-                                        hipMemcpyDeviceToDevice, h->stream), "hipMemcpyAsync"))
-            return EUP_EHIP;
-    }
-    return 0;
+    const Geometry &g = h->g;
+    double *d0 = static_cast<double *>(h->buf[0]) + (int64_t)(q - 1) * h->stride + h->poff;
+    double *d1 = static_cast<double *>(h->buf[1]) + (int64_t)(q - 1) * h->stride + h->poff;
+    hipLaunchKernelGGL(k_parity_put, dim3(nblocks(g.ghosted, 256)), dim3(256), 0, h->stream, h->staging, d0, d1,
+                       g.l, g.m, g.n, h->ppitch);
+    return hip_check(h, hipGetLastError(), "set pops") ? EUP_EHIP : 0;
 }
 
 int parity_get_pop(Handle *h, int q)
 {
-    const double *src = static_cast<const double *>(h->buf[h->cur]) + (int64_t)(q - 1) * h->stride;
-    return hip_check(h, hipMemcpyAsync(h->staging, src, sizeof(double) * h->g.ghosted,
-                                       hipMemcpyDeviceToDevice, h->stream), "hipMemcpyAsync")
-               ? EUP_EHIP
-               : 0;
+    const Geometry &g = h->g;
+    const double *src = static_cast<const double *>(h->buf[h->cur]) + (int64_t)(q - 1) * h->stride + h->poff;
+    hipLaunchKernelGGL(k_parity_take, dim3(nblocks(g.ghosted, 256)), dim3(256), 0, h->stream, src, h->staging,
+                       g.l, g.m, g.n, h->ppitch);
+    return hip_check(h, hipGetLastError(), "get pops") ? EUP_EHIP : 0;
 }
 
 } // namespace eup

@@ -367,7 +367,7 @@ __device__ __forceinline__ double hash_uniform(uint64_t seed, uint64_t i)
 }
 
 // equilibrium start over nsites sites; site t is (t % l, (t / l) % m, t / (l m)) of the layout P
-// (the parity layout passes its ghosted sites as one row: l = nsites, m = 1)
+// (the parity layout passes its ghosted extents l + 2, m + 2)
 template <typename S>
 __global__ void k_init_eq(S *base, PopIndex P, int l, int m, int64_t nsites, double ux, double uy0, double uz,
                           double noise, uint64_t seed, DevRef ref)
@@ -402,7 +402,8 @@ __global__ void k_outlet_refresh(XArgs<S> a, typename Store<S>::compute *o0,
     const int j = (int)(t % a.m), k = (int)(t / a.m);
     PopIndex P;
     P.stride = 0;
-    P.l = a.l;
+    P.off = 0;
+    P.pitch = a.l;
     P.m = a.m;
     P.tpr = a.tpr;
     P.tiled = 1;
@@ -457,6 +458,14 @@ static size_t fused_lds_bytes(const Handle *h)
 {
     int cap = h->elem == 2 ? 0 : (h->elem == 4 ? 5 : 3);
     if (const char *e = getenv("EUP_FUSED_BLOCKS_PER_CU")) cap = atoi(e);
+    return occupancy_lds_bytes(h, cap);
+}
+
+} // namespace
+
+// dynamic LDS per block that limits a kernel without LDS to `cap` blocks per CU (0 = no limit)
+size_t occupancy_lds_bytes(const Handle *h, int cap)
+{
     if (cap <= 0) return 0;
     int per_cu = 0;
     if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, h->cfg.device) !=
@@ -469,6 +478,8 @@ static size_t fused_lds_bytes(const Handle *h)
     if (b > (size_t)per_block) b = (size_t)per_block & ~(size_t)1023;
     return b;
 }
+
+namespace {
 
 template <typename S> int step_t(Handle *h)
 {
@@ -561,7 +572,7 @@ int init_equilibrium(Handle *h, double ux, double uy, double uz, double noise, u
     const int64_t ns = parity ? h->g.ghosted : h->g.sites;
     const unsigned grid = (unsigned)((ns + 255) / 256);
     const PopIndex P = pop_index(h);
-    const int il = parity ? (int)h->g.ghosted : h->g.l, im = parity ? 1 : h->g.m;
+    const int il = parity ? h->g.l + 2 : h->g.l, im = parity ? h->g.m + 2 : h->g.m;
     for (int b = 0; b < 2; ++b) {
         int rc = dispatch(h, [&](auto *p) {
             using S = std::remove_pointer_t<decltype(p)>;
