@@ -427,7 +427,7 @@ def main(argv=None):
                             else (None, None))
     cpu = None if args.no_cpu else cpu_baseline(wl)
     out = {
-        "metric": "MLUPS + % HBM peak, FP32 vs FP16-storage at |dC_D|<=1e-3 vs FP64",
+        "metric": "MLUPS + % HBM peak, FP32 vs FP16-storage at |\u0394C_D|\u22641e-3 vs FP64",
         "value": round(prim["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
