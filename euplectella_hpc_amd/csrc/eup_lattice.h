@@ -3,7 +3,8 @@
 // Direction numbering and weights: CODE/SRC/bgk3d.F90:31-49, storage.F90:77-81.  The collision,
 // inlet and outlet formulas are written in the reference's evaluation order (col.F90:127-229,
 // bcond.bc.F90:138-164 and :300-314); compiled with -ffp-contract=off the FP64 instantiation is
-// bit-identical to the reference's serial path.
+// bit-identical to the reference's serial path.  collide_dev is the same collision rearranged
+// for single precision (explicit fused multiply-adds) on deviation-form populations.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -137,6 +138,80 @@ EUP_HD void collide(const R *x, R *out, R omega, R fgrad)
     out[18] = omega1 * x[18] + (omega * e18);
 }
 
+// Constants of the single-precision collision on deviations (collide_dev).  The populations are
+// held as d_q = f_q - r_q with a constant r_q per population (r_q = 0 for full-value storage).
+struct BgkF {
+    float r[19];   // r_q
+    float wr[19];  // omega * r_q
+    float bb[19];  // r_opp(q) - r_q: added when population q bounces back from the site's own opp(q)
+    float rsum;    // sum_q r_q
+    float jr[3];   // sum_q c_q r_q
+    float omega1;  // 1 - omega
+    float ow0, ow1, ow2; // omega * p0, omega * p1, omega * p2
+    float fgrad;
+};
+
+// The BGK collision of col.F90:127-229 in value, written for single precision with fewer
+// operations, applied to deviations d[19] in place:
+//   rho = sum d + sum r, j = sum c d + sum c r, u = j / rho (hardware reciprocal, 1 ulp),
+//   omega e_q = W (s + 3 c_q.u) with W = omega w_q rho and s = 1 - 1.5 u^2 + 4.5 (c_q.u)^2, which
+//   is shared by q and its opposite (whose 3 c.u has the other sign),
+//   d'_q = (1 - omega) d_q + (omega e_q - omega r_q) +/- fgrad rho (c_x = +/-1, FORCING_X).
+// The moments are summed by planes of c_x (the x-momentum is the difference of two of them).
+// The FP64 path keeps the reference's own evaluation order (collide); this form is for the
+// FP32-compute storages, whose results are checked against FP64 with tolerances.
+template <bool DEV>
+__device__ __forceinline__ void collide_dev(float *d, const BgkF &c)
+{
+    const float sp = ((d[0] + d[1]) + (d[2] + d[3])) + d[4];     // c_x = +1
+    const float sm = ((d[9] + d[10]) + (d[11] + d[12])) + d[13]; // c_x = -1
+    const float s0 = (((d[5] + d[6]) + (d[7] + d[8])) + ((d[14] + d[15]) + (d[16] + d[17]))) + d[18];
+    float rho = (sp + sm) + s0;
+    float jx = sp - sm;
+    float jy = (((d[2] + d[6]) + (d[7] + d[8])) + d[11]) - (((d[0] + d[9]) + (d[15] + d[16])) + d[17]);
+    float jz = (((d[3] + d[5]) + (d[6] + d[12])) + d[17]) - (((d[1] + d[8]) + (d[10] + d[14])) + d[15]);
+    if constexpr (DEV) {
+        rho += c.rsum;
+        jx += c.jr[0];
+        jy += c.jr[1];
+        jz += c.jr[2];
+    }
+    const float rinv = __builtin_amdgcn_rcpf(rho);
+    const float ux = jx * rinv, uy = jy * rinv, uz = jz * rinv;
+    const float base = __builtin_fmaf(-1.5f, __builtin_fmaf(ux, ux, __builtin_fmaf(uy, uy, uz * uz)), 1.0f);
+    const float x3 = 3.0f * ux, y3 = 3.0f * uy, z3 = 3.0f * uz;
+    const float xh = 1.5f * ux, yh = 1.5f * uy, zh = 1.5f * uz;
+    const float W0 = c.ow0 * rho, W1 = c.ow1 * rho, W2 = c.ow2 * rho;
+    const float fx = c.fgrad * rho;
+    const float o1 = c.omega1;
+    // pair (q, qb = opp(q)) with t3 = 3 c_q.u, th = 1.5 c_q.u; F = forcing of q (-F for qb)
+    auto pair = [&](int q, int qb, float W, float t3, float th, float F, bool forced) {
+        const float s = __builtin_fmaf(t3, th, base);
+        float A, Ab;
+        if constexpr (DEV) {
+            A = __builtin_fmaf(W, s, -c.wr[q]);
+            Ab = __builtin_fmaf(W, s, -c.wr[qb]);
+        } else {
+            A = W * s;
+            Ab = A;
+        }
+        const float B = forced ? __builtin_fmaf(W, t3, F) : W * t3;
+        d[q] = __builtin_fmaf(o1, d[q], A + B);
+        d[qb] = __builtin_fmaf(o1, d[qb], Ab - B);
+    };
+    pair(0, 11, W2, x3 - y3, xh - yh, fx, true);  // (1,-1, 0)
+    pair(1, 12, W2, x3 - z3, xh - zh, fx, true);  // (1, 0,-1)
+    pair(2, 9, W2, x3 + y3, xh + yh, fx, true);   // (1, 1, 0)
+    pair(3, 10, W2, x3 + z3, xh + zh, fx, true);  // (1, 0, 1)
+    pair(4, 13, W1, x3, xh, fx, true);            // (1, 0, 0)
+    pair(5, 14, W1, z3, zh, 0.0f, false);         // (0, 0, 1)
+    pair(6, 15, W2, y3 + z3, yh + zh, 0.0f, false); // (0, 1, 1)
+    pair(7, 16, W1, y3, yh, 0.0f, false);         // (0, 1, 0)
+    pair(8, 17, W2, y3 - z3, yh - zh, 0.0f, false); // (0, 1,-1)
+    const float A18 = DEV ? __builtin_fmaf(W0, base, -c.wr[18]) : W0 * base;
+    d[18] = __builtin_fmaf(o1, d[18], A18);
+}
+
 // outlet ghost values o[0..4] = q10..q14 from the momentum (not divided by rho) of x[19];
 // bcond.bc.F90:138-164
 template <typename R>
@@ -178,6 +253,25 @@ EUP_HD void inlet_eq(R u_in, R *o)
     o[4] = W<R>::p1 * (one + c5);
 }
 
+// Division of n < 2^31 by a fixed divisor 1 <= d < 2^31 as a multiply and a shift:
+// with s = ceil(log2 d) and M = floor(2^(31+s) / d) + 1 (< 2^32), n / d = (n M) >> (31 + s).
+// Exact because M - 2^(31+s)/d lies in (0, 1] and n < 2^31 <= 2^(31+s)/d, so the error term
+// n (M - 2^(31+s)/d) / 2^(31+s) stays below 1/d.  Uniform operands compile to three scalar
+// instructions instead of a reciprocal-based integer division.
+struct UDiv {
+    uint32_t mul, shift;
+    static UDiv make(uint32_t d)
+    {
+        uint32_t s = 0;
+        while ((1ull << s) < d) ++s;
+        UDiv u;
+        u.mul = (uint32_t)(((1ull << (31 + s)) / d) + 1ull);
+        u.shift = 31 + s;
+        return u;
+    }
+    EUP_HD uint32_t div(uint32_t n) const { return (uint32_t)(((uint64_t)n * mul) >> shift); }
+};
+
 // ---------------------------------------------------------------------------------------------
 // storage traits: how a population is held in HBM and converted to the collision type
 // ---------------------------------------------------------------------------------------------
@@ -194,6 +288,8 @@ template <> struct Store<double> {
     static constexpr bool deviation = false;
     static __device__ __forceinline__ double load(double v, float) { return v; }
     static __device__ __forceinline__ double store(double f, float) { return f; }
+    static __device__ __forceinline__ double raw_load(double v) { return v; }
+    static __device__ __forceinline__ double raw_store(double f) { return f; }
 };
 
 template <> struct Store<float> {
@@ -201,6 +297,8 @@ template <> struct Store<float> {
     static constexpr bool deviation = false;
     static __device__ __forceinline__ float load(float v, float) { return v; }
     static __device__ __forceinline__ float store(float f, float) { return f; }
+    static __device__ __forceinline__ float raw_load(float v) { return v; }
+    static __device__ __forceinline__ float raw_store(float f) { return f; }
 };
 
 // deviation f - ref in binary16; float -> half conversion rounds to nearest even
@@ -209,6 +307,9 @@ template <> struct Store<_Float16> {
     static constexpr bool deviation = true;
     static __device__ __forceinline__ float load(_Float16 v, float ref) { return ref + (float)v; }
     static __device__ __forceinline__ _Float16 store(float f, float ref) { return (_Float16)(f - ref); }
+    // the stored deviation itself (kernels that compute on deviations)
+    static __device__ __forceinline__ float raw_load(_Float16 v) { return (float)v; }
+    static __device__ __forceinline__ _Float16 raw_store(float d) { return (_Float16)d; }
 };
 
 // deviation f - ref in bfloat16 with round-to-nearest-even
@@ -216,16 +317,18 @@ template <> struct Store<bf16_t> {
     using compute = float;
     static constexpr bool deviation = true;
     static __device__ __forceinline__ float to_float(bf16_t v) { return __uint_as_float((uint32_t)v.bits << 16); }
+    // hardware conversion (v_cvt_pk_bf16_f32, round to nearest even in the default mode)
     static __device__ __forceinline__ bf16_t from_float(float f)
     {
-        uint32_t u = __float_as_uint(f);
-        uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+        const __bf16 h = (__bf16)f;
         bf16_t b;
-        b.bits = (uint16_t)(r >> 16);
+        b.bits = __builtin_bit_cast(uint16_t, h);
         return b;
     }
     static __device__ __forceinline__ float load(bf16_t v, float ref) { return ref + to_float(v); }
     static __device__ __forceinline__ bf16_t store(float f, float ref) { return from_float(f - ref); }
+    static __device__ __forceinline__ float raw_load(bf16_t v) { return to_float(v); }
+    static __device__ __forceinline__ bf16_t raw_store(float d) { return from_float(d); }
 };
 
 } // namespace eup

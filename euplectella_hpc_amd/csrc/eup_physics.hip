@@ -38,13 +38,28 @@ template <typename S> struct XArgs {
     const uint32_t *link;
     const R *out_cur; // 5 x (m*n)
     R *out_next;
-    R inl[5];
-    DevRef ref;
+    R inl[5];    // inlet ghost values, in the kernel's representation (deviations for FP32 compute)
+    DevRef ref;  // deviation references of the storage (conversions to and from full values)
+    BgkF bgk;    // constants of the FP32-compute collision (unused by FP64)
     R omega, fgrad;
     int l, m, n, tpr;
+    UDiv dtpr, dm; // t / tpr and row / m (t < tiles < 2^31)
     uint32_t tiles;
     uint32_t chunk; // logical blocks per XCD
+    uint32_t remap; // 1: XCD-contiguous block order, 0: dispatch order
 };
+
+// FP32-compute storages (FP32, FP16 and BF16 deviations) run the step on the stored values
+// themselves: deviations d_q = f_q - r_q, with r_q = 0 for full-value FP32 (collide_dev).  FP64
+// computes on full values in the reference's evaluation order (collide).
+template <typename S> constexpr bool kOnStored = std::is_same_v<typename Store<S>::compute, float>;
+
+template <typename S>
+__device__ __forceinline__ typename Store<S>::compute ld(const XArgs<S> &a, const S *p, int q)
+{
+    if constexpr (kOnStored<S>) return Store<S>::raw_load(*p);
+    else return Store<S>::load(*p, a.ref.r[q]);
+}
 
 template <int... Qs> struct QSeq {};
 using Q18 = QSeq<0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17>;
@@ -111,15 +126,29 @@ __device__ __forceinline__ void pull(const XArgs<S> &a, const Wave &w, uint32_t 
     if constexpr (cx == 1) p = at<S>(rb, QB - TB, w.op);
     else if constexpr (cx == 0) p = at<S>(rb, QB, w.o0);
     else p = at<S>(rb, QB, w.om);
-    float ref = a.ref.r[Q];
-    if constexpr (MIXED) {
-        // the source is solid: bounce back the site's own opposite population
-        if ((lm >> Q) & 1u) {
-            p = at<S>(w.row[1][1], (int64_t)op * kTile * sizeof(S), w.o0);
-            ref = a.ref.r[op];
+    if constexpr (kOnStored<S>) {
+        float corr = 0.0f;
+        if constexpr (MIXED) {
+            // the source is solid: bounce back the site's own opposite population (as a deviation
+            // from r_Q it carries r_opp - r_Q on top of the stored deviation)
+            if ((lm >> Q) & 1u) {
+                p = at<S>(w.row[1][1], (int64_t)op * kTile * sizeof(S), w.o0);
+                corr = a.bgk.bb[Q];
+            }
         }
+        if constexpr (MIXED && Store<S>::deviation) x[Q] = Store<S>::raw_load(*p) + corr;
+        else x[Q] = Store<S>::raw_load(*p);
+    } else {
+        float ref = a.ref.r[Q];
+        if constexpr (MIXED) {
+            // the source is solid: bounce back the site's own opposite population
+            if ((lm >> Q) & 1u) {
+                p = at<S>(w.row[1][1], (int64_t)op * kTile * sizeof(S), w.o0);
+                ref = a.ref.r[op];
+            }
+        }
+        x[Q] = Store<S>::load(*p, ref);
     }
-    x[Q] = Store<S>::load(*p, ref);
 }
 
 template <typename S, bool MIXED, int... Qs>
@@ -150,9 +179,9 @@ __device__ __forceinline__ void x_edges(const XArgs<S> &a, const Wave &w, uint32
             } else {
                 if (edge && !((lm >> Q) & 1u)) {
                     const char *rb = w.row[1 - cys[Q]][1 - czs[Q]];
-                    x[Q] = Store<S>::load(*at<S>(rb, (int64_t)Q * kTile * sizeof(S) + (int64_t)(a.tpr - 1) * TB,
-                                                 (uint32_t)((a.l - 1) & 63) * (uint32_t)sizeof(S)),
-                                          a.ref.r[Q]);
+                    x[Q] = ld<S>(a, at<S>(rb, (int64_t)Q * kTile * sizeof(S) + (int64_t)(a.tpr - 1) * TB,
+                                          (uint32_t)((a.l - 1) & 63) * (uint32_t)sizeof(S)),
+                                 Q);
                 }
             }
         }
@@ -162,14 +191,15 @@ __device__ __forceinline__ void x_edges(const XArgs<S> &a, const Wave &w, uint32
 #pragma unroll
         for (int Q = 9; Q < 14; ++Q) {
             if constexpr (XOPEN) {
-                const R o = a.out_cur[(Q - 9) * (size_t)(a.m * a.n) + w.jj[1 - cys[Q]] +
-                                      (size_t)a.m * w.kk[1 - czs[Q]]];
+                // wave-uniform address into a plane the launch only reads: a scalar load
+                typedef const __attribute__((address_space(4))) R ConstR;
+                const R o = *(ConstR *)(a.out_cur + ((Q - 9) * (size_t)(a.m * a.n) + w.jj[1 - cys[Q]] +
+                                                     (size_t)a.m * w.kk[1 - czs[Q]]));
                 x[Q] = edge ? o : x[Q];
             } else {
                 if (edge && !((lm >> Q) & 1u)) {
                     const char *rb = w.row[1 - cys[Q]][1 - czs[Q]];
-                    x[Q] = Store<S>::load(*at<S>(rb, (int64_t)Q * kTile * sizeof(S) - (int64_t)w.ti * TB, 0u),
-                                          a.ref.r[Q]);
+                    x[Q] = ld<S>(a, at<S>(rb, (int64_t)Q * kTile * sizeof(S) - (int64_t)w.ti * TB, 0u), Q);
                 }
             }
         }
@@ -197,7 +227,8 @@ template <typename S, int Q>
 __device__ __forceinline__ void put(const XArgs<S> &a, char *tb, uint32_t o0, const typename Store<S>::compute *x)
 {
     S *p = reinterpret_cast<S *>(tb + (int64_t)Q * kTile * sizeof(S) + (uint64_t)o0);
-    store_val<S>(p, Store<S>::store(x[Q], a.ref.r[Q]));
+    if constexpr (kOnStored<S>) store_val<S>(p, Store<S>::raw_store(x[Q]));
+    else store_val<S>(p, Store<S>::store(x[Q], a.ref.r[Q]));
 }
 
 template <typename S, int... Qs>
@@ -220,15 +251,26 @@ __device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, const
     }
     R x[19];
     pull_all<S, MIXED>(Q18{}, a, w, lm, x);
-    x[18] = Store<S>::load(*at<S>(w.row[1][1], 18 * kTile * (int64_t)sizeof(S), w.o0), a.ref.r[18]);
+    x[18] = ld<S>(a, at<S>(w.row[1][1], 18 * kTile * (int64_t)sizeof(S), w.o0), 18);
     x_edges<S, XOPEN>(a, w, lm, x);
     if (w.i >= a.l) return; // padding lanes of a partial last tile
-    collide<R>(x, x, a.omega, a.fgrad);
+    if constexpr (kOnStored<S>) collide_dev<Store<S>::deviation>(x, a.bgk);
+    else collide<R>(x, x, a.omega, a.fgrad);
     put_all<S>(Q19{}, a, reinterpret_cast<char *>(a.B) + (int64_t)t * TB, w.o0, x);
     if constexpr (XOPEN) {
         if (w.i == a.l - 1) {
             R o[5];
-            outlet_eq<R>(x, o);
+            if constexpr (kOnStored<S> && Store<S>::deviation) {
+                // the outlet formula takes full values; the ghost plane holds deviations
+                R xf[19];
+#pragma unroll
+                for (int q = 0; q < 19; ++q) xf[q] = x[q] + a.bgk.r[q];
+                outlet_eq<R>(xf, o);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) o[v] -= a.bgk.r[9 + v];
+            } else {
+                outlet_eq<R>(x, o);
+            }
             const size_t mn = (size_t)a.m * a.n;
 #pragma unroll
             for (int v = 0; v < 5; ++v) a.out_next[v * mn + j + (size_t)a.m * k] = o[v];
@@ -236,13 +278,19 @@ __device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, const
     }
 }
 
+// No register target beyond the block size: asking the compiler for 8 wavefronts per SIMD (64
+// registers) makes it interleave the 19 loads with their uses and costs the 2-byte storages
+// 7-13 % on MI355X, although it removes only one register.
 template <typename S, bool XOPEN>
 __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
 {
-    // XCD-aware block order: hardware block b runs on XCD b % 8; give each XCD a contiguous
-    // range of tiles so that the lines shared by neighbouring tiles stay in one L2.
+    // Block order.  Hardware block b runs on XCD b % 8.  With remap each XCD gets a contiguous
+    // range of tiles (the lines shared by neighbouring tiles stay in one L2); without it the eight
+    // XCDs sweep the lattice together in dispatch order.  Measured on MI355X (512 x 256 x 256
+    // sponge): FP64 runs about 5 % faster with the remap, FP32 and the 2-byte storages about 2 %
+    // faster without it.
     const uint32_t b = blockIdx.x;
-    const uint32_t lb = (b & 7u) * a.chunk + (b >> 3);
+    const uint32_t lb = a.remap ? (b & 7u) * a.chunk + (b >> 3) : b;
     const uint32_t t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kTile) + (threadIdx.x >> 6));
     if (t >= a.tiles) return;
     // the codes are read-only during the launch: a scalar load through the constant address space
@@ -250,9 +298,10 @@ __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
     // the addressing below does not depend on the code and overlaps the load
     typedef const __attribute__((address_space(4))) int32_t ConstI32;
     const int32_t code = *(ConstI32 *)(a.tile_code + t);
-    const uint32_t row = t / (uint32_t)a.tpr;
+    const uint32_t row = a.dtpr.div(t);
     const int ti = (int)(t - row * (uint32_t)a.tpr);
-    const int j = (int)(row % (uint32_t)a.m), k = (int)(row / (uint32_t)a.m);
+    const uint32_t kk = a.dm.div(row);
+    const int j = (int)(row - kk * (uint32_t)a.m), k = (int)kk;
     Wave w;
     wave_setup<S>(a, w, ti, j, k, (int)(threadIdx.x & 63));
     if (code == kTileSolid) return;
@@ -413,8 +462,10 @@ __global__ void k_outlet_refresh(XArgs<S> a, typename Store<S>::compute *o0,
     outlet_eq<R>(x, o);
 #pragma unroll
     for (int u = 0; u < 5; ++u) {
-        o0[u * mn + t] = o[u];
-        o1[u * mn + t] = o[u];
+        // in the step kernel's representation (r = 0 for full values)
+        const R v = kOnStored<S> ? o[u] - (R)a.bgk.r[9 + u] : o[u];
+        o0[u * mn + t] = v;
+        o1[u * mn + t] = v;
     }
 }
 
@@ -428,20 +479,56 @@ template <typename S> XArgs<S> make_args(Handle *h)
     a.link = h->link_mask;
     a.out_cur = static_cast<const R *>(h->outlet[h->cur]);
     a.out_next = static_cast<R *>(h->outlet[h->cur ^ 1]);
-    R inl[5];
-    inlet_eq<R>((R)h->cfg.u_inflow, inl);
-    for (int u = 0; u < 5; ++u) a.inl[u] = inl[u];
     a.ref = dev_ref(h);
     a.omega = (R)h->cfg.omega;
     a.fgrad = (R)h->cfg.fgrad;
+    // r_q of the FP32-compute step: the storage's deviation references, or 0 for full values
+    static const int opp[19] = {11, 12, 9, 10, 13, 14, 15, 16, 17, 2, 3, 0, 1, 4, 5, 6, 7, 8, 18};
+    static const int cx[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+    static const int cy[19] = {-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1, 0};
+    static const int cz[19] = {0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1, 0};
+    double r[19];
+    for (int q = 0; q < 19; ++q) r[q] = Store<S>::deviation ? (double)a.ref.r[q] : 0.0;
+    const double om = h->cfg.omega;
+    BgkF &c = a.bgk;
+    double rsum = 0.0, jr[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < 19; ++q) {
+        c.r[q] = (float)r[q];
+        c.wr[q] = (float)(om * r[q]);
+        c.bb[q] = (float)(r[opp[q]] - r[q]);
+        rsum += r[q];
+        jr[0] += cx[q] * r[q];
+        jr[1] += cy[q] * r[q];
+        jr[2] += cz[q] * r[q];
+    }
+    c.rsum = (float)rsum;
+    for (int d = 0; d < 3; ++d) c.jr[d] = (float)jr[d];
+    c.omega1 = (float)(1.0 - om);
+    c.ow0 = (float)(om / 3.0);
+    c.ow1 = (float)(om / 18.0);
+    c.ow2 = (float)(om / 36.0);
+    c.fgrad = (float)h->cfg.fgrad;
+    if constexpr (kOnStored<S>) {
+        double inl[5];
+        inlet_eq<double>(h->cfg.u_inflow, inl);
+        for (int u = 0; u < 5; ++u) a.inl[u] = (R)(inl[u] - r[u]);
+    } else {
+        R inl[5];
+        inlet_eq<R>((R)h->cfg.u_inflow, inl);
+        for (int u = 0; u < 5; ++u) a.inl[u] = inl[u];
+    }
     a.l = h->g.l;
     a.m = h->g.m;
     a.n = h->g.n;
     a.tpr = h->g.tiles_per_row;
+    a.dtpr = UDiv::make((uint32_t)a.tpr);
+    a.dm = UDiv::make((uint32_t)a.m);
     a.tiles = (uint32_t)h->g.tiles;
     const uint32_t per_block = kBlock / kTile;
     const uint32_t nb = (uint32_t)((h->g.tiles + per_block - 1) / per_block);
     a.chunk = (nb + 7) / 8;
+    a.remap = sizeof(S) == 8 ? 1u : 0u; // EUP_FUSED_XCD_REMAP=0/1 overrides
+    if (const char *e = getenv("EUP_FUSED_XCD_REMAP")) a.remap = atoi(e) != 0;
     return a;
 }
 

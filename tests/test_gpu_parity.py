@@ -121,6 +121,10 @@ def physics_case(kind, l, m, n, seed, u=0.05):
         mask = geometry.cylinder(l, m, n, max(2, m // 5), (l // 4, m // 2))
     elif kind == "sponge":
         mask = geometry.sponge(l, m, n, m // 4, (l // 4, m // 2), cell=8, seed=seed)
+    elif kind == "block":
+        # a solid block covering whole 64-site tiles (all-solid tiles) next to a cylinder
+        mask = geometry.cylinder(l, m, n, max(2, m // 6), (l // 5, m // 2))
+        mask[:, 2:m - 3, 64:128] = 1
     else:
         mask = np.zeros((n, m, l), np.uint8)
     pops = geometry.initial_state(l, m, n, u, seed=seed)
@@ -131,13 +135,19 @@ def interior(p):
     return p[:, 1:-1, 1:-1, 1:-1]
 
 
+@pytest.mark.parametrize("remap", ["0", "1"])
 @pytest.mark.parametrize("kind,x_open,shape", [("cylinder", True, (70, 24, 20)),
                                                 ("cylinder", False, (64, 24, 20)),
                                                 ("sponge", False, (100, 24, 20)),
-                                                ("sponge", True, (128, 40, 48))])
-def test_physics_f64_equals_oracle(kind, x_open, shape):
+                                                ("sponge", True, (128, 40, 48)),
+                                                ("block", True, (200, 24, 20)),
+                                                ("block", False, (192, 24, 20))])
+def test_physics_f64_equals_oracle(kind, x_open, shape, remap, monkeypatch):
+    """The fused kernel in both block orders (XCD-contiguous or dispatch order) against the
+    oracle, bit for bit."""
+    monkeypatch.setenv("EUP_FUSED_XCD_REMAP", remap)
     l, m, n = shape
-    announce(f"physics f64 {kind} {shape} x_open={x_open} vs oracle, 25 steps")
+    announce(f"physics f64 {kind} {shape} x_open={x_open} xcd_remap={remap} vs oracle, 25 steps")
     mask, pops = physics_case(kind, l, m, n, seed=7)
     svisc = (2 * 0.55 - 1) / 6
     ref = pops.copy()
@@ -148,6 +158,8 @@ def test_physics_f64_equals_oracle(kind, x_open, shape):
         lat.set_state(pops)
         lat.step(25)
         got = lat.get_state()
+        if kind == "block":
+            assert lat.info()["tiles_solid"] > 0
     a, b = interior(got), interior(ref)
     bad = a.view(np.uint64) != b.view(np.uint64)
     assert not bad.any(), f"{bad.sum()} values differ, max abs {np.abs(a - b).max()}"
@@ -155,24 +167,29 @@ def test_physics_f64_equals_oracle(kind, x_open, shape):
 
 @pytest.mark.parametrize("shape,x_open", [((128, 48, 48), True), ((200, 24, 24), False)])
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
-def test_physics_tile_skip_is_exact(storage, shape, x_open):
-    """The pure-fluid tile path against the per-site mixed path (every tile marked mixed)."""
+def test_physics_tile_skip_is_exact(storage, shape, x_open, monkeypatch):
+    """The pure-fluid tile path against the per-site mixed path (every tile marked mixed), in
+    both block orders."""
     l, m, n = shape
-    announce(f"physics {storage} sponge {l}x{m}x{n} x_open={x_open}: tile skip vs no skip")
     mask, pops = physics_case("sponge", l, m, n, seed=3)
     svisc = (2 * 0.52 - 1) / 6
     out = []
-    for skip in (True, False):
-        with lattice((l, m, n), svisc=svisc, u_inflow=0.06, mode="physics", storage=storage,
-                     tile_skip=skip, x_open=x_open) as lat:
-            lat.set_mask(mask)
-            lat.set_state(pops)
-            lat.step(20)
-            out.append(lat.get_state())
-            if skip:
-                info = lat.info()
-                assert info["tiles_mixed"] > 0 and info["tiles_fluid"] > 0
-    assert np.array_equal(out[0], out[1])
+    for remap in ("0", "1"):
+        monkeypatch.setenv("EUP_FUSED_XCD_REMAP", remap)
+        for skip in (True, False):
+            announce(f"physics {storage} sponge {l}x{m}x{n} x_open={x_open} xcd_remap={remap} "
+                     f"tile_skip={skip}")
+            with lattice((l, m, n), svisc=svisc, u_inflow=0.06, mode="physics", storage=storage,
+                         tile_skip=skip, x_open=x_open) as lat:
+                lat.set_mask(mask)
+                lat.set_state(pops)
+                lat.step(20)
+                out.append(lat.get_state())
+                if skip:
+                    info = lat.info()
+                    assert info["tiles_mixed"] > 0 and info["tiles_fluid"] > 0
+    for o in out[1:]:
+        assert np.array_equal(out[0], o)
 
 
 @pytest.mark.parametrize("storage,tol,shifted", [("f32", 2e-6, False), ("f16", 2e-4, False),
