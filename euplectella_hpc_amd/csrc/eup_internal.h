@@ -61,6 +61,7 @@ struct Handle {
     void *outlet[2] = {nullptr, nullptr}; // outlet ghost plane, 5 x m x n (compute type)
     bool outlet_dirty = true;
     size_t fused_lds = (size_t)-1; // dynamic LDS per block of the fused kernel (occupancy cap)
+    int fused_remap = -1;          // fused kernel block order: 1 XCD-contiguous, 0 dispatch order
     size_t parity_lds = (size_t)-1; // the same for the parity stream-collide kernel
     // diagnostics scratch
     double *diag = nullptr;       // device scratch for reductions

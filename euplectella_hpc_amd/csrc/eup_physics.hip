@@ -360,11 +360,9 @@ template <typename S> constexpr bool kMixedAtOnce = sizeof(S) <= 4;
 template <typename S, bool XOPEN>
 __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
 {
-    // Block order.  Hardware block b runs on XCD b % 8.  With remap each XCD gets a contiguous
-    // range of tiles (the lines shared by neighbouring tiles stay in one L2); without it the eight
-    // XCDs sweep the lattice together in dispatch order.  Measured on MI355X (512 x 256 x 256
-    // sponge): FP64 runs about 5 % faster with the remap, FP32 and the 2-byte storages about 2 %
-    // faster without it.
+    // Block order (chosen by fused_block_order).  Hardware block b runs on XCD b % 8.  With remap
+    // each XCD gets a contiguous range of tiles; without it the eight XCDs sweep the lattice
+    // together in dispatch order.
     const uint32_t b = blockIdx.x;
     const uint32_t lb = a.remap ? (b & 7u) * a.chunk + (b >> 3) : b;
     const uint32_t t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kTile) + (threadIdx.x >> 6));
@@ -605,8 +603,7 @@ template <typename S> XArgs<S> make_args(Handle *h)
     const uint32_t per_block = kBlock / kTile;
     const uint32_t nb = (uint32_t)((h->g.tiles + per_block - 1) / per_block);
     a.chunk = (nb + 7) / 8;
-    a.remap = sizeof(S) == 8 ? 1u : 0u; // EUP_FUSED_XCD_REMAP=0/1 overrides
-    if (const char *e = getenv("EUP_FUSED_XCD_REMAP")) a.remap = atoi(e) != 0;
+    a.remap = h->fused_remap > 0 ? 1u : 0u; // set by the step launch (fused_block_order)
     return a;
 }
 
@@ -619,12 +616,33 @@ inline bool x_open(const Handle *h) { return h->cfg.front != EUP_FACE_PERIODIC; 
 // wavefront.  Measured on MI355X (512 x 256 x 256 sponge): FP32 runs 2-8 % faster at 5 blocks per
 // CU than uncapped (8, set by its registers), FP64 about 3 % faster at 3; the 2-byte storages,
 // whose loads move 128 B, are fastest uncapped (0).  EUP_FUSED_BLOCKS_PER_CU overrides the choice.
-static size_t fused_lds_bytes(const Handle *h)
+static int fused_cap(const Handle *h)
 {
     int cap = h->elem == 2 ? 0 : (h->elem == 4 ? 5 : 3);
     if (const char *e = getenv("EUP_FUSED_BLOCKS_PER_CU")) cap = atoi(e);
-    return occupancy_lds_bytes(h, cap);
+    return cap;
 }
+
+// Block order of the fused kernel.  In dispatch order the resident wavefronts of the whole GPU
+// form one window that moves through the lattice; when that window covers at least two planes of
+// tiles (z = k - 1, k, k + 1 are then resident together), the rows a tile pulls from are fetched
+// while their other readers are resident too, and dispatch order is faster.  Otherwise each XCD
+// is given a contiguous range of tiles.  Measured on MI355X (window / plane of tiles, dispatch
+// order against remap): 512 x 256 x 256 sponge FP32 (2.5) +2.3 %, FP16 (4.0) +2.2 %, FP64 (1.5)
+// -5 %; 1536 x 768 x 768 sponge FP32 (0.28) -7 %, FP16 (0.44) -4 %; its 192 x 768 x 768 slab
+// FP32 (2.2) +8 %; 256 x 128 x 128 FP32 (10) +13 %.  EUP_FUSED_XCD_REMAP=0/1 forces the choice.
+static int fused_block_order(const Handle *h, int cap)
+{
+    if (const char *e = getenv("EUP_FUSED_XCD_REMAP")) return atoi(e) != 0;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->cfg.device) != hipSuccess || cus <= 0)
+        cus = 256;
+    const double window = (double)cus * (cap > 0 ? cap : 8) * (kBlock / kTile);
+    const double plane = (double)h->g.tiles_per_row * h->g.m;
+    return window >= 2.0 * plane ? 0 : 1;
+}
+
+static size_t fused_lds_bytes(const Handle *h) { return occupancy_lds_bytes(h, fused_cap(h)); }
 
 } // namespace
 
@@ -648,6 +666,7 @@ namespace {
 
 template <typename S> int step_t(Handle *h)
 {
+    if (h->fused_remap < 0) h->fused_remap = fused_block_order(h, fused_cap(h));
     XArgs<S> a = make_args<S>(h);
     const unsigned grid = 8u * a.chunk; // multiple of 8 so the XCD remap is a bijection
     if (h->fused_lds == (size_t)-1) h->fused_lds = fused_lds_bytes(h);
