@@ -6,6 +6,12 @@ little-endian length marker before and after the payload.
 * deck ``bgk.input``: namelist ``&parameters`` (CODE/SRC/input.F90:46-62)
 * ``obs.RRRR.TTTTTTTT.bin``: record ``nobs`` (int32), then ``nobs`` records ``i, j, k`` (int32,
   1-based) (reader CODE/SRC/build.obs.F90:147-186, writer :506-531)
+* ``restore.obs.bin``: the HDF5-free equivalent of the reference's ``restore.obs.h5`` (datasets
+  ``/time/timestep``, ``/size/dimension``, ``/obstacles/obs``; reader
+  CODE/SRC/restore.obs.hdf5.serial.F90:85-160, writer save.obs.hdf5.serial.F90): record ``itime``
+  (int32), record ``l, m, n`` (int32), then one record per z plane k = 1..n of ``l*m`` reals (FP64
+  or FP32, i fastest).  As in the reference, the reals become the integer obstacle field by
+  truncation and a site is solid where that integer is 1.
 * ``restore.XX.YY.ZZ.bin`` / ``save.XX.YY.ZZ.TTTTTTTT.bin``: record ``itime`` (int32), then 19
   records, population q = 1..19, each ``(l+2)(m+2)(n+2)`` reals with i fastest
   (CODE/SRC/restore_raw.F90:43-63, CODE/SRC/save_raw.F90:54-74)
@@ -112,6 +118,99 @@ def read_obs(path, shape):
         cnt += 1
     if cnt != nobs:
         raise ValueError(f"{path}: header says {nobs} obstacles, found {cnt}")
+    return mask
+
+
+# ---------------------------------------------------------------------------------------------
+# dense obstacle field (HDF5-free restore.obs.h5)
+# ---------------------------------------------------------------------------------------------
+
+def obs_dense_file_name():
+    return "restore.obs.bin"
+
+
+def write_obs_dense(path, values, itime=0, dtype="<f8"):
+    """Write the obstacle field ``values[k, j, i]`` (shape (n, m, l); 1 = solid, as reals) with
+    its time step and dimensions, one record per z plane."""
+    values = np.asarray(values)
+    if values.ndim != 3:
+        raise ValueError("obstacle field must be 3-D [n, m, l]")
+    n, m, l = values.shape
+
+    def records():
+        yield np.array([itime], dtype="<i4")
+        yield np.array([l, m, n], dtype="<i4")
+        for k in range(n):
+            yield np.ascontiguousarray(values[k], dtype=dtype)
+
+    write_records(path, records())
+
+
+def _read_record(fh, path):
+    head = fh.read(4)
+    if len(head) != 4:
+        raise ValueError(f"{path}: truncated dense obstacle file")
+    (nb,) = _MARK.unpack(head)
+    if nb < 0:
+        raise ValueError(f"{path}: sub-record markers are not supported")
+    data = fh.read(nb)
+    tail = fh.read(4)
+    if len(data) != nb or len(tail) != 4 or _MARK.unpack(tail)[0] != nb:
+        raise ValueError(f"{path}: corrupt record framing")
+    return data
+
+
+def iter_obs_dense(path, shape, chunk=1 << 20):
+    """The solid sites of a dense obstacle file as int32 [rows, 3] blocks of 1-based (i, j, k),
+    at most ``chunk`` rows each, reading one z plane at a time.  ``shape`` is (l, m, n); a file
+    of other dimensions is refused, as the reference stops on a size mismatch
+    (restore.obs.hdf5.serial.F90:123-136).  Returns the file's time step through the generator's
+    return value."""
+    l, m, n = shape
+    with open(path, "rb") as fh:
+        itime = struct.unpack("<i", _read_record(fh, path))[0]
+        dims = np.frombuffer(_read_record(fh, path), dtype="<i4")
+        if dims.size != 3 or tuple(int(d) for d in dims) != (l, m, n):
+            raise ValueError(f"{path}: size doesn't match: file {tuple(dims)}, lattice {(l, m, n)}")
+        pending = []
+        rows = 0
+        for k in range(n):
+            data = _read_record(fh, path)
+            if len(data) == 8 * l * m:
+                vals = np.frombuffer(data, dtype="<f8")
+            elif len(data) == 4 * l * m:
+                vals = np.frombuffer(data, dtype="<f4")
+            else:
+                raise ValueError(f"{path}: plane {k + 1} holds {len(data)} bytes, not l*m reals")
+            # Fortran real -> integer assignment truncates toward zero; solid where it gives 1
+            solid = np.trunc(vals) == 1
+            jj, ii = np.nonzero(solid.reshape(m, l))
+            if ii.size:
+                trip = np.empty((ii.size, 3), dtype=np.int32)
+                trip[:, 0] = ii + 1
+                trip[:, 1] = jj + 1
+                trip[:, 2] = k + 1
+                pending.append(trip)
+                rows += ii.size
+            while rows >= chunk:
+                allr = np.concatenate(pending)
+                yield allr[:chunk]
+                rest = allr[chunk:]
+                pending = [rest] if rest.size else []
+                rows = rest.shape[0]
+        if rows:
+            yield np.concatenate(pending)
+        if fh.read(1):
+            raise ValueError(f"{path}: data after the last plane")
+    return itime
+
+
+def read_obs_dense(path, shape):
+    """Read a dense obstacle file into a mask ``[n, m, l]`` (uint8)."""
+    l, m, n = shape
+    mask = np.zeros((n, m, l), dtype=np.uint8)
+    for trip in iter_obs_dense(path, shape):
+        mask[trip[:, 2] - 1, trip[:, 1] - 1, trip[:, 0] - 1] = 1
     return mask
 
 

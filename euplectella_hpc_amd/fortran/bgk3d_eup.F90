@@ -3,6 +3,10 @@
 ! Keeps the reference's files and formats unchanged:
 !   bgk.input               namelist &parameters (CODE/SRC/input.F90:46-62)
 !   obs.0000.00000000.bin   obstacle list, flag1 = 1 (build.obs.F90:147-186)
+!   restore.obs.bin         with EUP_OBS=dense and flag1 = 1: the dense obstacle field in place of
+!                           the list, the HDF5-free form of the -DHDF5 build's restore.obs.h5
+!                           (restore.obs.hdf5.serial.F90): records itime, (l, m, n), then one
+!                           record of l*m FP64 values per z plane; truncated to integer, 1 = solid
 !   restore.00.00.00.bin    raw restart, irestart = 1 (restore_raw.F90:43-63)
 !   save.00.00.00.T.bin     raw dump every itsave steps (save_raw.F90:54-74)
 !   drag.0000.dat           control-volume drag every icheck steps (drag.F90:48-67), written when
@@ -43,6 +47,8 @@ program bgk3d_eup
    integer(c_int) :: rc
    character(len=32) :: env
    character(len=27) :: fname
+   integer :: dims(3), itobs
+   real(c_double), allocatable :: plane(:)
 
    flag1 = 2; flag2 = 2; flag3 = 1; ipad = 0; jpad = 0; kpad = 0
    open(15, file='bgk.input', status='old')
@@ -94,7 +100,41 @@ program bgk3d_eup
 
    ! obstacles (build_obs, flag1 = 1 reads the task's list; 0 = no obstacle): the list is streamed
    ! to the device in chunks of at most 2^20 records, no dense obs array on the host
-   if (flag1 == 1) then
+   call get_environment_variable('EUP_OBS', env)
+   if (flag1 == 1 .and. trim(env) == 'dense') then
+      ! the dense field, one z plane at a time; its solid sites go through the same list path
+      open(54, file='restore.obs.bin', form='unformatted', status='old')
+      read(54) itobs
+      read(54) dims
+      if (dims(1) /= l .or. dims(2) /= m .or. dims(3) /= n) then
+         write(6,*) "ERROR: size doesn't match", l, m, n, dims
+         stop 1
+      end if
+      allocate(plane(l * m), chunk(3, 1048576))
+      cnt = 0
+      do k = 1, n
+         read(54) plane
+         do j = 1, m
+            do i = 1, l
+               if (int(plane(i + l * (j - 1))) == 1) then
+                  cnt = cnt + 1
+                  chunk(1, cnt) = i; chunk(2, cnt) = j; chunk(3, cnt) = k
+                  if (cnt == size(chunk, 2)) then
+                     rc = eup_add_obstacles(h, chunk, int(cnt, c_int64_t))
+                     call eup_check(rc, h, 'eup_add_obstacles')
+                     cnt = 0
+                  end if
+               end if
+            end do
+         end do
+      end do
+      if (cnt > 0) then
+         rc = eup_add_obstacles(h, chunk, int(cnt, c_int64_t))
+         call eup_check(rc, h, 'eup_add_obstacles')
+      end if
+      close(54)
+      deallocate(plane, chunk)
+   else if (flag1 == 1) then
       open(53, file='obs.0000.00000000.bin', form='unformatted', status='old')
       read(53) nobs
       allocate(chunk(3, max(1, min(nobs, 1048576))))

@@ -222,6 +222,21 @@ class Lattice:
             self.add_obstacles(block)
         self.commit_obstacles()
 
+    def set_obstacle_dense_file(self, path, chunk=1 << 20):
+        """Stream a dense obstacle field (restore.obs.bin, the HDF5-free form of the reference's
+        restore.obs.h5) into the lattice one z plane at a time: its solid sites go through
+        eup_add_obstacles, so no dense mask is held on the host.  Returns the file's time step."""
+        it = fortio.iter_obs_dense(path, self.shape, chunk)
+        while True:
+            try:
+                block = next(it)
+            except StopIteration as stop:
+                itime = stop.value
+                break
+            self.add_obstacles(block)
+        self.commit_obstacles()
+        return itime
+
     def set_state(self, pops):
         l, m, n = self.shape
         p = np.ascontiguousarray(pops, dtype=np.float64)

@@ -49,3 +49,45 @@ def test_deck_round_trip(tmp_path):
     back = fortio.read_deck(path)
     for key, value in fields.items():
         assert back[key] == value, key
+
+
+def test_dense_obstacle_field_streams_solid_sites_by_truncation(tmp_path):
+    """restore.obs.bin (the HDF5-free restore.obs.h5): the reals become the integer obstacle field
+    by truncation toward zero and a site is solid where that integer is 1, as the reference's
+    reader assigns the real buffer to its integer obs array and tests obs == 1
+    (restore.obs.hdf5.serial.F90:160, build.obs.F90:114-118)."""
+    l, m, n = 13, 7, 5
+    rng = np.random.default_rng(4)
+    vals = rng.choice(np.array([0.0, 1.0, 0.999, 1.5, 1.999, 2.0, -0.5, -1.0]), size=(n, m, l))
+    want = np.trunc(vals) == 1
+    assert want.any() and (~want).any()
+    for dtype in ("<f8", "<f4"):
+        path = str(tmp_path / f"obs_{dtype[-1]}.bin")
+        fortio.write_obs_dense(path, vals, itime=42, dtype=dtype)
+        for chunk in (1, 5, 1 << 20):
+            it = fortio.iter_obs_dense(path, (l, m, n), chunk)
+            blocks = []
+            while True:
+                try:
+                    blocks.append(next(it))
+                except StopIteration as stop:
+                    assert stop.value == 42
+                    break
+            assert all(0 < b.shape[0] <= chunk for b in blocks)
+            got = np.concatenate(blocks)
+            k, j, i = np.nonzero(want)
+            assert np.array_equal(got, np.stack([i + 1, j + 1, k + 1], axis=1))
+        assert np.array_equal(fortio.read_obs_dense(path, (l, m, n)) != 0, want)
+
+
+def test_dense_obstacle_field_refuses_other_dimensions(tmp_path):
+    import pytest
+    path = str(tmp_path / fortio.obs_dense_file_name())
+    fortio.write_obs_dense(path, np.zeros((4, 3, 5)))
+    assert fortio.read_obs_dense(path, (5, 3, 4)).sum() == 0
+    with pytest.raises(ValueError, match="size doesn't match"):
+        fortio.read_obs_dense(path, (5, 3, 5))
+    with open(path, "ab") as fh:
+        fh.write(b"\0")
+    with pytest.raises(ValueError):
+        fortio.read_obs_dense(path, (5, 3, 4))

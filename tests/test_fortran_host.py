@@ -71,6 +71,28 @@ def test_host_parity_run_matches_reference_files(golden):
 
 
 @pytest.mark.gpu
+def test_host_dense_obstacle_field_matches_reference_files(golden):
+    """EUP_OBS=dense: the host reads restore.obs.bin (the HDF5-free restore.obs.h5) instead of
+    the obstacle list and writes the reference's files unchanged."""
+    rec = golden["c1_small"]
+    mask, pops = make_inputs(CASES["c1_small"])
+    print("[gpu] Fortran host, parity mode, c1_small 10 steps, dense obstacle field", flush=True)
+    with tempfile.TemporaryDirectory() as wd:
+        write_case(wd, rec, mask, pops, icheck=1, itsave=5)
+        os.remove(os.path.join(wd, fortio.obs_file_name()))
+        fortio.write_obs_dense(os.path.join(wd, fortio.obs_dense_file_name()),
+                               mask.astype(np.float64))
+        res = run_host(wd, {"EUP_MODE": "parity", "EUP_OBS": "dense"})
+        assert res.returncode == 0, res.stdout + res.stderr
+        with open(os.path.join(wd, "probe_g.0000.dat")) as fh:
+            assert fh.read() == rec["probe_text"]
+        _, state = fortio.read_raw(os.path.join(wd, fortio.save_file_name(10)),
+                                   (rec["n"], rec["m"], rec["l"]))
+        digest = hashlib.sha256(np.ascontiguousarray(state).tobytes()).hexdigest()
+        assert digest == rec["state_sha256"]["10"]
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("branch", ["default", "hpc"])
 def test_host_vtk_files_match_reference(golden, branch):
     """ivtim = the run length: the host writes the reference's binary VTK files of the chosen

@@ -291,6 +291,33 @@ def test_streamed_obstacle_list_equals_dense_mask(mode, storage, tmp_path):
     assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
 
 
+@pytest.mark.parametrize("mode", ["parity", "physics"])
+def test_dense_obstacle_file_equals_dense_mask(mode, tmp_path):
+    """Lattice.set_obstacle_dense_file (restore.obs.bin streamed plane by plane through
+    eup_add_obstacles) gives the same solid count, bounding box and states as eup_set_mask."""
+    from euplectella_hpc_amd import fortio
+    l, m, n = 70, 24, 20
+    announce(f"{mode} sponge {l}x{m}x{n}: dense obstacle file vs dense mask")
+    mask, pops = physics_case("sponge", l, m, n, seed=22)
+    path = str(tmp_path / fortio.obs_dense_file_name())
+    fortio.write_obs_dense(path, mask.astype(np.float64), itime=3)
+    out, infos = [], []
+    for from_file in (False, True):
+        with lattice((l, m, n), svisc=(2 * 0.6 - 1) / 6, u_inflow=0.05, mode=mode,
+                     storage="f64") as lat:
+            if from_file:
+                assert lat.set_obstacle_dense_file(path, chunk=131) == 3
+            else:
+                lat.set_mask(mask)
+            lat.set_state(pops)
+            lat.step(10)
+            out.append(lat.get_state())
+            info = lat.info()
+            infos.append((info["nsolid"], tuple(info["bbox"])))
+    assert infos[0] == infos[1] and infos[0][0] == int(mask.sum())
+    assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
+
+
 def test_obstacle_list_commit_replace_and_range_check():
     from euplectella_hpc_amd.lib import EupError
     l, m, n = 40, 16, 12
