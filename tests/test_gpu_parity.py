@@ -121,6 +121,9 @@ def physics_case(kind, l, m, n, seed, u=0.05):
         mask = geometry.cylinder(l, m, n, max(2, m // 5), (l // 4, m // 2))
     elif kind == "sponge":
         mask = geometry.sponge(l, m, n, m // 4, (l // 4, m // 2), cell=8, seed=seed)
+    elif kind == "random":
+        # scattered solid sites: most tiles mixed, some sites with every neighbour solid
+        mask = (np.random.default_rng(seed).random((n, m, l)) < 0.2).astype(np.uint8)
     elif kind == "block":
         # a solid block covering whole 64-site tiles (all-solid tiles) next to a cylinder
         mask = geometry.cylinder(l, m, n, max(2, m // 6), (l // 5, m // 2))
@@ -141,7 +144,10 @@ def interior(p):
                                                 ("sponge", False, (100, 24, 20)),
                                                 ("sponge", True, (128, 40, 48)),
                                                 ("block", True, (200, 24, 20)),
-                                                ("block", False, (192, 24, 20))])
+                                                ("block", False, (192, 24, 20)),
+                                                ("random", True, (5, 3, 2)),
+                                                ("random", False, (67, 2, 3)),
+                                                ("random", True, (130, 5, 4))])
 def test_physics_f64_equals_oracle(kind, x_open, shape, remap, monkeypatch):
     """The fused kernel in both block orders (XCD-contiguous or dispatch order) against the
     oracle, bit for bit."""
