@@ -39,8 +39,12 @@ def parity_lattice(name, golden):
     return rec, mask, pops, lat
 
 
-def test_parity_tiny_full_state(golden):
-    announce("parity c0_tiny 16x8x12, 20 steps")
+@pytest.mark.parametrize("remap", ["0", "1"])
+def test_parity_tiny_full_state(golden, remap, monkeypatch):
+    """Every site of the state (ghosts included) after 20 steps, bit for bit, with the
+    stream-collide in dispatch order and with each XCD on a contiguous range of segments."""
+    monkeypatch.setenv("EUP_PARITY_XCD_REMAP", remap)
+    announce(f"parity c0_tiny 16x8x12, 20 steps, xcd_remap={remap}")
     rec, mask, pops, lat = parity_lattice("c0_tiny", golden)
     with lat:
         lat.step(rec["steps"])
@@ -50,8 +54,10 @@ def test_parity_tiny_full_state(golden):
     assert not bad.any(), f"{bad.sum()} values differ (first at {np.argwhere(bad)[:3].tolist()})"
 
 
-@pytest.mark.parametrize("name", ["c1_small", "c2_ridged", "c3_sponge", "c1_cylinder"])
-def test_parity_state_hashes(golden, name):
+@pytest.mark.parametrize("name,remap", [("c1_small", "0"), ("c2_ridged", "0"), ("c3_sponge", "0"),
+                                         ("c1_cylinder", "0"), ("c3_sponge", "1")])
+def test_parity_state_hashes(golden, name, remap, monkeypatch):
+    monkeypatch.setenv("EUP_PARITY_XCD_REMAP", remap)
     rec, mask, pops, lat = parity_lattice(name, golden)
     announce(f"parity {name} {rec['l']}x{rec['m']}x{rec['n']} up to {rec['steps']} steps")
     done = 0
