@@ -11,11 +11,15 @@
 // every load and store of a population is a contiguous 64-lane access (256 B in FP32, 128 B in
 // FP16) whose address is a wave-uniform row pointer (scalar registers), one of three lane offsets
 // (x shift -1, 0, +1) and the constant population offset Q * 64 values.  Per tile a 4-byte code
-// says whether the tile is pure
-// fluid (no solid site in or next to it: no mask traffic at all), all solid (skipped: no loads, no
-// stores), or mixed (the 64 per-site link masks are read: bit q set = the source of population q
-// is solid, bit 31 = the site itself is solid).  The codes are built once per mask with a
-// wavefront ballot (k_classify).
+// says whether the tile is pure fluid (no solid site in or next to it: no mask traffic at all),
+// all solid (skipped: no loads, no stores), or mixed (the 64 per-site link masks are read: bit q
+// set = the source of population q is solid, bit 31 = the site itself is solid).  The codes are
+// built once per mask with a wavefront ballot (k_classify).
+//
+// Arithmetic: FP64 storage collides full values in the reference's evaluation order (collide,
+// bit-identical to the oracle); FP32, FP16 and BF16 storage collide the stored values themselves
+// (deviations from a constant reference per population, zero for FP32) in single precision
+// (collide_dev).
 //
 // Boundaries in x: the inlet ghost (x = 0) is the constant equilibrium at (u_in, 0, 0); the outlet
 // ghost plane (x = l+1, 5 populations) is written by the sites at x = l while they collide and
