@@ -180,23 +180,32 @@ def _torch_sync():
         pass
 
 
-def cpu_baseline(wl, steps=30, nz=64):
-    """The reference's OpenMP path on a 512 x 256 x 64 slab of the workload (bounded sample)."""
+def cpu_baseline(wl, steps=50, nz_slab=64, mask=None):
+    """The reference's OpenMP path timed on a bounded sample of the workload: the whole lattice
+    when the reference's FP64 arrays fit in 16 GB of host memory (about 10 s of CPU work for the
+    512 x 256 x 256 sponge at 50 steps), else a slab of ``nz_slab`` z planes."""
     from euplectella_hpc_amd.lib import svisc_from_tau
     from oracle import refrun
 
     if refrun.ref_binary("omp") is None:
         return None
-    l, m = wl["l"], wl["m"]
-    if l * m * nz > 512 * 256 * 64:
-        return None  # the FP64 reference would not fit a bounded sample of this workload
-    mask = workload_mask(wl, nz)
+    l, m, n = wl["l"], wl["m"], wl["n"]
+    # a and b, 19 populations each, ghosted FP64 (CODE/SRC/alloca.F90:94-133)
+    def footprint(nz):
+        return 38 * (l + 2) * (m + 2) * (nz + 2) * 8
+    nz, what = n, f"the whole {l}x{m}x{n} lattice of the workload"
+    if footprint(n) > 16e9:
+        nz, what = nz_slab, f"a {l}x{m}x{nz_slab} slab of the {l}x{m}x{n} workload"
+        if footprint(nz) > 16e9:
+            return None  # the FP64 reference would not fit a bounded sample of this workload
+    if nz != n or mask is None:
+        mask = workload_mask(wl, nz)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     r = refrun.time_reference(l, m, nz, svisc_from_tau(wl["tau"]), wl["u_in"], steps, mask,
                               threads=threads)
     return {"value": r["mlups"], "unit": "MLUPS", "cores": threads, "kind": "reference",
-            "sample": f"reference OpenMP FP64 (amdflang -O2), {l}x{m}x{nz} slab of the same "
-                      f"sponge, {steps} steps, MLUPS from its bgk.perf"}
+            "sample": f"reference OpenMP FP64 (amdflang -O2) on {what}, {steps} steps, "
+                      "MLUPS from its bgk.perf"}
 
 
 def copy_probe(mib, reps=10):
@@ -425,7 +434,7 @@ def main(argv=None):
     streams = None if args.no_streams else hbm_streams()
     traffic, traffic_src = (profiled_traffic(args.workload, args.storage) if args.mode == "physics"
                             else (None, None))
-    cpu = None if args.no_cpu else cpu_baseline(wl)
+    cpu = None if args.no_cpu else cpu_baseline(wl, mask=mask)
     out = {
         "metric": "MLUPS + % HBM peak, FP32 vs FP16-storage at |\u0394C_D|\u22641e-3 vs FP64",
         "value": round(prim["mlups"], 1),
