@@ -242,6 +242,11 @@ int eup_create(const eup_config *cfg, eup_handle **out)
         for (int b = 0; b < 2 && !rc; ++b)
             if (hip_check(h, hipMalloc(&h->outlet[b], rb * 5 * (size_t)g.m * g.n), "hipMalloc outlet")) rc = EUP_EHIP;
     }
+    // parity mode: the outlet equilibria of the x = l sites, written by the stream-collide for the
+    // next step's ghost fill (FP64, 5 x m x n)
+    if (!rc && cfg->mode == EUP_MODE_PARITY &&
+        hip_check(h, hipMalloc(&h->outlet[0], sizeof(double) * 5 * (size_t)g.m * g.n), "hipMalloc outlet"))
+        rc = EUP_EHIP;
     if (!rc && cfg->mode == EUP_MODE_GREY &&
         hip_check(h, hipMemset(h->solid, 0, g.sites), "hipMemset solid"))
         rc = EUP_EHIP;

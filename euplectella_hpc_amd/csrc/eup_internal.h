@@ -59,7 +59,8 @@ struct Handle {
     int64_t tiles_fluid = 0, tiles_mixed = 0, tiles_solid = 0;
     int32_t *mixed_list = nullptr; // tile indices of mixed tiles
     void *outlet[2] = {nullptr, nullptr}; // outlet ghost plane, 5 x m x n (compute type)
-    bool outlet_dirty = true;
+    bool outlet_dirty = true;      // the host wrote populations: physics mode recomputes its outlet
+                                   // plane, parity mode refills the constant inflow plane
     size_t fused_lds = (size_t)-1; // dynamic LDS per block of the fused kernel (occupancy cap)
     int fused_remap = -1;          // fused kernel block order: 1 XCD-contiguous, 0 dispatch order
     size_t parity_lds = (size_t)-1; // the same for the parity stream-collide kernel

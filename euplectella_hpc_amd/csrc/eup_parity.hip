@@ -35,6 +35,8 @@ struct PArgs {
     int pitch;        // row pitch (elements) of the ghosted layout
     int l, m, n;
     double omega, fgrad, u_in;
+    double *outp;     // outlet equilibria of the x = l sites, 5 x m x n (written by the stream-collide)
+    const uint8_t *solid;
 };
 
 __device__ __forceinline__ int64_t gidx(int lp, int mp, int i, int j, int k)
@@ -107,9 +109,14 @@ __device__ __forceinline__ void put2(const PArgs &a, int q, int64_t s, double v)
 }
 
 // flattened work: [0, P) x = l+1 plane, [P, 2P) x = 0 plane, then l*m z-face columns, then
-// l*n y-face rows; P = (m+2)(n+2)
-__global__ void k_ghost(PArgs a)
+// l*n y-face rows; P = (m+2)(n+2).  The inflow values (x = 0, populations 1..5) are constants that
+// nothing else writes except the corner copy a03(0,0,k) of the y fill, so they are written only
+// when `inflow` is set (the first step after the host set the state); the corner copy is done
+// every step, and so every cell ends each step with the value the reference leaves in it.
+__global__ void k_ghost(PArgs a, int inflow)
 {
+    // inflow = 1 also means that the outlet plane was not written by a stream-collide since the
+    // host last set the state: the outflow values are then computed here from A
     const int l = a.l, m = a.m, n = a.n, lp = a.pitch, mp = m + 2;
     const int64_t P = (int64_t)(m + 2) * (n + 2);
     const int64_t Z = (int64_t)l * m, Y = (int64_t)l * n;
@@ -120,11 +127,20 @@ __global__ void k_ghost(PArgs a)
         if (!jin && !kin) return; // corners of the plane: never written
         const int sj = jin ? j : (j == 0 ? 1 : m);     // border fix sources (:171-204)
         const int sk = kin ? k : (k == 0 ? 1 : n);
-        const int64_t src = gidx(lp, mp, l, sj, sk);
-        double x[19], o[5];
+        double o[5];
+        // a solid site at x = l had its populations replaced by this step's bounce-back after the
+        // stream-collide computed the plane: recompute those from A
+        const int64_t mn = (int64_t)m * n, pl = (int64_t)(sj - 1) + (int64_t)m * (sk - 1);
+        if (!inflow && a.solid[(int64_t)(l - 1) + (int64_t)l * pl] == 0) {
 #pragma unroll
-        for (int q = 0; q < 19; ++q) x[q] = a.A[q][src];
-        outlet_eq<double>(x, o);
+            for (int u = 0; u < 5; ++u) o[u] = a.outp[u * mn + pl];
+        } else {
+            const int64_t src = gidx(lp, mp, l, sj, sk);
+            double x[19];
+#pragma unroll
+            for (int q = 0; q < 19; ++q) x[q] = a.A[q][src];
+            outlet_eq<double>(x, o);
+        }
         const int64_t d = gidx(lp, mp, l + 1, j, k);
 #pragma unroll
         for (int u = 0; u < 5; ++u) put2(a, 9 + u, d, o[u]);
@@ -134,11 +150,13 @@ __global__ void k_ghost(PArgs a)
     t -= P;
     if (t < P) { // inflow plane x = 0
         const int j = (int)(t % (m + 2)), k = (int)(t / (m + 2));
-        double o[5];
-        inlet_eq<double>(a.u_in, o);
         const int64_t d = gidx(lp, mp, 0, j, k);
+        if (inflow) {
+            double o[5];
+            inlet_eq<double>(a.u_in, o);
 #pragma unroll
-        for (int u = 0; u < 5; ++u) put2(a, u, d, o[u]);
+            for (int u = 0; u < 5; ++u) put2(a, u, d, o[u]);
+        }
         if (j == 0 && k >= 1 && k <= n) put2(a, 2, d, a.A[2][gidx(lp, mp, l, m, k)]); // :547
         return;
     }
@@ -221,6 +239,15 @@ __global__ __launch_bounds__(256) void k_stream_collide(PArgs a, SegMap s)
 #pragma unroll
     for (int q = 0; q < 19; ++q)
         __builtin_nontemporal_store(x[q], &(a.B0 + ((int64_t)q * a.stride + row))[(uint64_t)(uint32_t)i]);
+    if (i == a.l) {
+        // the next step's outflow ghost values of this row (bcond.bc.F90:127-164), from the values
+        // just stored; the ghost fill uses them unless bounce-back changes this site first
+        double o[5];
+        outlet_eq<double>(x, o);
+        const int64_t mn = (int64_t)a.m * a.n, pl = (int64_t)(j - 1) + (int64_t)a.m * (k - 1);
+#pragma unroll
+        for (int u = 0; u < 5; ++u) a.outp[u * mn + pl] = o[u];
+    }
 }
 
 // one population between the reference's ghosted array (row pitch l + 2; the staging buffer) and
@@ -267,6 +294,8 @@ PArgs make_args(Handle *h, int from)
     a.omega = h->cfg.omega;
     a.fgrad = h->cfg.fgrad;
     a.u_in = h->cfg.u_inflow;
+    a.outp = static_cast<double *>(h->outlet[0]);
+    a.solid = h->solid;
     return a;
 }
 
@@ -327,7 +356,9 @@ int parity_step(Handle *h)
                            h->solid_list, h->nsolid, h->bb_tmp);
     }
     const int64_t gwork = 2 * (int64_t)(g.m + 2) * (g.n + 2) + (int64_t)g.l * g.m + (int64_t)g.l * g.n;
-    hipLaunchKernelGGL(k_ghost, dim3(nblocks(gwork, 256)), dim3(256), 0, h->stream, a);
+    // the host's set / init calls mark the ghosts dirty (outlet_dirty): fill the inflow plane
+    hipLaunchKernelGGL(k_ghost, dim3(nblocks(gwork, 256)), dim3(256), 0, h->stream, a, h->outlet_dirty ? 1 : 0);
+    h->outlet_dirty = false;
     SegMap sm;
     sm.tpr = (g.l + 63) / 64;
     sm.segments = (uint32_t)((int64_t)sm.tpr * g.m * g.n);

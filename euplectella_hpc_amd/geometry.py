@@ -141,6 +141,14 @@ def sponge(l, m, n, radius, centre, cell=16, thickness=3.0, seed=0, ridges=0, ri
     return mask
 
 
+def scattered(l, m, n, fraction=0.1, seed=0):
+    """Seeded scattered solid sites over the whole lattice (uint8, shape (n, m, l)), the first and
+    last x planes and the y/z domain edges included: the boundary cases of bounce-back next to the
+    inlet and outlet ghosts and across the periodic seams."""
+    rng = np.random.default_rng(seed)
+    return (rng.random((n, m, l)) < fraction).astype(np.uint8)
+
+
 def grey_spheres(l, m, n, n_spheres=12, rmin=4.0, rmax=8.0, sigma=1.5, seed=0, region=None):
     """Grey-lattice solid fraction s in [0, 1] (float64, shape (n, m, l))."""
     from scipy.ndimage import gaussian_filter

@@ -48,6 +48,9 @@ CASES = {
                       geom=("sponge", dict(radius=12, centre=(24, 24), cell=8, thickness=3.0,
                                            seed=4)),
                       tau=0.505, u=0.06, seed=4, steps=200, saves=(100, 200), probe=True),
+    # solid sites everywhere, on the inlet and outlet columns and the periodic edges included
+    "c4_scattered": dict(l=24, m=12, n=10, geom=("scattered", dict(fraction=0.12, seed=9)),
+                         tau=0.6, u=0.05, seed=9, steps=20, saves=(1, 2, 3, 10, 20), probe=False),
     "drag_r100": dict(l=680, m=340, n=4, geom=("cylinder", dict(radius=100, centre=(170, 170))),
                       tau=0.6, u=0.05, seed=5, steps=3, saves=(3,), probe=False, drag=True),
 }

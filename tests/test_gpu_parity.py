@@ -55,7 +55,8 @@ def test_parity_tiny_full_state(golden, remap, monkeypatch):
 
 
 @pytest.mark.parametrize("name,remap", [("c1_small", "0"), ("c2_ridged", "0"), ("c3_sponge", "0"),
-                                         ("c1_cylinder", "0"), ("c3_sponge", "1")])
+                                         ("c1_cylinder", "0"), ("c3_sponge", "1"),
+                                         ("c4_scattered", "0"), ("c4_scattered", "1")])
 def test_parity_state_hashes(golden, name, remap, monkeypatch):
     monkeypatch.setenv("EUP_PARITY_XCD_REMAP", remap)
     rec, mask, pops, lat = parity_lattice(name, golden)

@@ -29,7 +29,7 @@ def test_oracle_tiny_full_state(golden):
 
 
 @pytest.mark.parametrize("name,upto", [("c1_small", 10), ("c2_ridged", 200), ("c3_sponge", 200),
-                                       ("c1_cylinder", 200)])
+                                       ("c1_cylinder", 200), ("c4_scattered", 20)])
 def test_oracle_state_hashes(golden, name, upto):
     rec = golden[name]
     mask, pops = make_inputs(CASES[name])
