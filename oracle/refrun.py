@@ -27,12 +27,16 @@ def ref_binary(variant="serial"):
 
 
 def run_reference(l, m, n, svisc, u0, steps, mask, pops0, *, variant="nodrag", itsave=None,
-                  icheck=None, ivtim=None, threads=1, workdir=None, keep=False, timeout=3600):
-    """Run ``steps`` time steps of the reference from the ghosted state ``pops0``.
+                  icheck=None, ivtim=None, threads=1, workdir=None, keep=False, timeout=3600,
+                  irestart=1):
+    """Run ``steps`` time steps of the reference from the ghosted state ``pops0`` (irestart = 1),
+    or from its own rest state (irestart = 0, init_v = 0: CODE/SRC/init.F90:128-236; ``pops0`` is
+    then ignored).
 
     Returns a dict with ``saves`` ({itime: pops}), ``drag`` (list of (delta, itime, fx, fy)),
     ``probe`` (list of (itime, u, v, w, rho)), ``vtk`` ({file name: bytes} of the binary VTK
-    files written every ``ivtim`` steps, CODE/SRC/diagnostic.F90:40-90), ``mlups`` and ``stdout``.
+    files written every ``ivtim`` steps, CODE/SRC/diagnostic.F90:40-90), ``diagno_text`` (the
+    diagno.dat written at a fresh start, CODE/SRC/initialize.F90:145), ``mlups`` and ``stdout``.
     """
     exe = ref_binary(variant)
     if exe is None:
@@ -44,10 +48,11 @@ def run_reference(l, m, n, svisc, u0, steps, mask, pops0, *, variant="nodrag", i
         fortio.write_deck(os.path.join(wd, "bgk.input"), svisc=float(svisc), u0=float(u0),
                           itfin=int(steps), ivtim=int(ivtim if ivtim else big), isignal=big,
                           itsave=int(itsave if itsave else big), icheck=int(icheck if icheck else big),
-                          irestart=1, init_v=0, lx=l, ly=m, lz=n, proc_x=1, proc_y=1, proc_z=1,
-                          tstep=1, flag1=1, flag2=2, flag3=0)
+                          irestart=int(irestart), init_v=0, lx=l, ly=m, lz=n, proc_x=1, proc_y=1,
+                          proc_z=1, tstep=1, flag1=1, flag2=2, flag3=0)
         fortio.write_obs(os.path.join(wd, fortio.obs_file_name()), mask)
-        fortio.write_raw(os.path.join(wd, fortio.restore_file_name()), 0, pops0)
+        if irestart:
+            fortio.write_raw(os.path.join(wd, fortio.restore_file_name()), 0, pops0)
         env = dict(os.environ)
         env["OMP_NUM_THREADS"] = str(threads)
         env.setdefault("OMP_PROC_BIND", "close")
@@ -72,6 +77,7 @@ def run_reference(l, m, n, svisc, u0, steps, mask, pops0, *, variant="nodrag", i
         out["probe"] = _read_table(os.path.join(wd, "probe_g.0000.dat"), 5)
         out["drag_text"] = _read_text(os.path.join(wd, "drag.0000.dat"))
         out["probe_text"] = _read_text(os.path.join(wd, "probe_g.0000.dat"))
+        out["diagno_text"] = _read_text(os.path.join(wd, "diagno.dat"))
         out["mlups"] = _read_mlups(os.path.join(wd, "bgk.perf"))
         return out
     finally:

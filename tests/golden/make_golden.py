@@ -48,6 +48,9 @@ CASES = {
                       geom=("sponge", dict(radius=12, centre=(24, 24), cell=8, thickness=3.0,
                                            seed=4)),
                       tau=0.505, u=0.06, seed=4, steps=200, saves=(100, 200), probe=True),
+    # the reference's own fresh start (irestart = 0, init_v = 0: the rest state over the ghosts)
+    "c1_fresh": dict(l=32, m=16, n=16, geom=("cylinder", dict(radius=4, centre=(8, 8))),
+                     tau=0.6, u=0.05, seed=1, steps=10, saves=(5, 10), probe=True, fresh=True),
     # solid sites everywhere, on the inlet and outlet columns and the periodic edges included
     "c4_scattered": dict(l=24, m=12, n=10, geom=("scattered", dict(fraction=0.12, seed=9)),
                          tau=0.6, u=0.05, seed=9, steps=20, saves=(1, 2, 3, 10, 20), probe=False),
@@ -80,10 +83,12 @@ def main(names=None):
         saves = case["saves"]
         step = saves[0] if len(saves) == 1 else int(np.gcd.reduce(np.array(saves)))
         vtk = case.get("vtk", False)
+        fresh = case.get("fresh", False)
         res = refrun.run_reference(l, m, n, svisc, case["u"], case["steps"], mask, pops,
                                    variant="serial" if drag else "nodrag", itsave=step,
                                    icheck=1 if (drag or case.get("probe")) else None,
-                                   ivtim=case["steps"] if vtk else None)
+                                   ivtim=case["steps"] if vtk else None,
+                                   irestart=0 if fresh else 1)
         rec = {
             "l": l, "m": m, "n": n, "geometry": [case["geom"][0], case["geom"][1]],
             "tau": case["tau"], "svisc": svisc, "u_in": case["u"], "seed": case["seed"],
@@ -96,6 +101,9 @@ def main(names=None):
             rec["probe_site"] = [8 * l // 20, 9 * m // 16, n // 2 - 5]
             rec["probe"] = [list(r) for r in res["probe"]]
             rec["probe_text"] = res["probe_text"]
+        if fresh:
+            rec["fresh"] = True
+            rec["diagno_text"] = res["diagno_text"]
         if drag:
             rec["drag"] = [list(r[:4]) for r in res["drag"]]
             rec["drag_text"] = res["drag_text"]

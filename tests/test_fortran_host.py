@@ -71,6 +71,39 @@ def test_host_parity_run_matches_reference_files(golden):
 
 
 @pytest.mark.gpu
+def test_host_fresh_start_matches_reference_files(golden):
+    """irestart = 0: the host starts from the rest state as the reference's init does and writes
+    the same save dumps (bit for bit), probe records (character for character) and diagno.dat
+    line; diagno's means are reductions whose summation order differs from the reference's serial
+    loop, so its values are compared to rounding (the rest state's mean velocity is itself
+    rounding noise of order 1e-17)."""
+    rec = golden["c1_fresh"]
+    mask, pops = make_inputs(CASES["c1_fresh"])
+    print("[gpu] Fortran host, parity mode, c1_fresh 10 steps from rest", flush=True)
+    with tempfile.TemporaryDirectory() as wd:
+        write_case(wd, rec, mask, pops, icheck=1, itsave=5, irestart=0)
+        os.remove(os.path.join(wd, fortio.restore_file_name()))
+        res = run_host(wd, {"EUP_MODE": "parity"})
+        assert res.returncode == 0, res.stdout + res.stderr
+        with open(os.path.join(wd, "probe_g.0000.dat")) as fh:
+            assert fh.read() == rec["probe_text"]
+        for t in (5, 10):
+            _, state = fortio.read_raw(os.path.join(wd, fortio.save_file_name(t)),
+                                       (rec["n"], rec["m"], rec["l"]))
+            digest = hashlib.sha256(np.ascontiguousarray(state).tobytes()).hexdigest()
+            assert digest == rec["state_sha256"][str(t)], f"save dump at step {t} differs"
+        with open(os.path.join(wd, "diagno.dat")) as fh:
+            got = fh.read()
+    want = rec["diagno_text"]
+    assert len(got) == len(want) and got.split()[0] == want.split()[0] == "0"
+    g = np.array([float(v) for v in got.split()[1:]])
+    w = np.array([float(v) for v in want.split()[1:]])
+    assert abs(g[3] - w[3]) <= 1e-13                      # mean rho
+    assert np.all(np.abs(g[:3] - w[:3]) <= 1e-15)         # mean velocity components
+    assert abs(g[4] - w[4]) <= 1e-30                      # mean |u|^2
+
+
+@pytest.mark.gpu
 def test_host_dense_obstacle_field_matches_reference_files(golden):
     """EUP_OBS=dense: the host reads restore.obs.bin (the HDF5-free restore.obs.h5) instead of
     the obstacle list and writes the reference's files unchanged."""

@@ -103,3 +103,21 @@ def test_oracle_vtk_files_match_reference(golden, branch):
     for name, data in files.items():
         assert len(data) == want[name]["bytes"], name
         assert hashlib.sha256(data).hexdigest() == want[name]["sha256"], name
+
+
+def test_oracle_fresh_start_equals_reference(golden):
+    """irestart = 0: the reference starts from its rest state a_q = w_q on every site, ghosts
+    included (init.F90:128-236); the oracle stepping the same state gives its save dumps."""
+    from euplectella_hpc_amd import d3q19
+    rec = golden["c1_fresh"]
+    mask, _ = make_inputs(CASES["c1_fresh"])
+    l, m, n = rec["l"], rec["m"], rec["n"]
+    pops = np.empty((19, n + 2, m + 2, l + 2))
+    for q in range(19):
+        pops[q] = d3q19.W[q]
+    done = 0
+    for t in sorted(int(s) for s in rec["state_sha256"]):
+        orc.parity_steps(pops, mask, omega_of(rec), rec["u_in"], t - done)
+        done = t
+        assert sha(pops) == rec["state_sha256"][str(t)], f"fresh start differs at step {t}"
+
