@@ -94,6 +94,10 @@ def test_host_fresh_start_matches_reference_files(golden):
             assert digest == rec["state_sha256"][str(t)], f"save dump at step {t} differs"
         with open(os.path.join(wd, "diagno.dat")) as fh:
             got = fh.read()
+        # bgk.perf carries the MLUPS line in the reference's format (finalize.F90:107-108,222),
+        # which the reference's own timing reader parses
+        from oracle import refrun
+        assert refrun._read_mlups(os.path.join(wd, "bgk.perf")) > 0
     want = rec["diagno_text"]
     assert len(got) == len(want) and got.split()[0] == want.split()[0] == "0"
     g = np.array([float(v) for v in got.split()[1:]])
