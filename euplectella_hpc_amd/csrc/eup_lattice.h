@@ -10,9 +10,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-namespace eup {
-
 #define EUP_HD __host__ __device__ __forceinline__
+
+#include "eup_udiv.h"
+
+namespace eup {
 
 // Direction table as compile-time constants, 0-based population index Q = 0..18 for the
 // reference's directions 1..19; opp = index of -c_Q.
@@ -252,25 +254,6 @@ EUP_HD void inlet_eq(R u_in, R *o)
     o[3] = W<R>::p2 * (one + c4);
     o[4] = W<R>::p1 * (one + c5);
 }
-
-// Division of n < 2^31 by a fixed divisor 1 <= d < 2^31 as a multiply and a shift:
-// with s = ceil(log2 d) and M = floor(2^(31+s) / d) + 1 (< 2^32), n / d = (n M) >> (31 + s).
-// Exact because M - 2^(31+s)/d lies in (0, 1] and n < 2^31 <= 2^(31+s)/d, so the error term
-// n (M - 2^(31+s)/d) / 2^(31+s) stays below 1/d.  Uniform operands compile to three scalar
-// instructions instead of a reciprocal-based integer division.
-struct UDiv {
-    uint32_t mul, shift;
-    static UDiv make(uint32_t d)
-    {
-        uint32_t s = 0;
-        while ((1ull << s) < d) ++s;
-        UDiv u;
-        u.mul = (uint32_t)(((1ull << (31 + s)) / d) + 1ull);
-        u.shift = 31 + s;
-        return u;
-    }
-    EUP_HD uint32_t div(uint32_t n) const { return (uint32_t)(((uint64_t)n * mul) >> shift); }
-};
 
 // ---------------------------------------------------------------------------------------------
 // storage traits: how a population is held in HBM and converted to the collision type
