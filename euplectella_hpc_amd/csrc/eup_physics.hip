@@ -242,6 +242,31 @@ __device__ __forceinline__ void put_all(QSeq<Qs...>, const XArgs<S> &a, char *tb
     (put<S, Qs>(a, tb, o0, x), ...);
 }
 
+// The x = l sites write the next step's outlet ghost values (bcond.bc.F90:127-164) from their
+// post-collision populations x, in the step kernel's representation.
+template <typename S>
+__device__ __forceinline__ void outlet_next(const XArgs<S> &a, const Wave &w, int j, int k,
+                                            const typename Store<S>::compute *x)
+{
+    using R = typename Store<S>::compute;
+    if (w.i != a.l - 1) return;
+    R o[5];
+    if constexpr (kOnStored<S> && Store<S>::deviation) {
+        // the outlet formula takes full values; the ghost plane holds deviations
+        R xf[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) xf[q] = x[q] + a.bgk.r[q];
+        outlet_eq<R>(xf, o);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) o[v] -= a.bgk.r[9 + v];
+    } else {
+        outlet_eq<R>(x, o);
+    }
+    const size_t mn = (size_t)a.m * a.n;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) a.out_next[v * mn + j + (size_t)a.m * k] = o[v];
+}
+
 // Update tile t (addressing w, row j, k), one site per lane.  MIXED applies the link mask lm of the
 // lane's site: bounce-back from solid sources, solid sites frozen.
 template <typename S, bool XOPEN, bool MIXED>
@@ -261,25 +286,7 @@ __device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, const
     if constexpr (kOnStored<S>) collide_dev<Store<S>::deviation>(x, a.bgk);
     else collide<R>(x, x, a.omega, a.fgrad);
     put_all<S>(Q19{}, a, reinterpret_cast<char *>(a.B) + (int64_t)t * TB, w.o0, x);
-    if constexpr (XOPEN) {
-        if (w.i == a.l - 1) {
-            R o[5];
-            if constexpr (kOnStored<S> && Store<S>::deviation) {
-                // the outlet formula takes full values; the ghost plane holds deviations
-                R xf[19];
-#pragma unroll
-                for (int q = 0; q < 19; ++q) xf[q] = x[q] + a.bgk.r[q];
-                outlet_eq<R>(xf, o);
-#pragma unroll
-                for (int v = 0; v < 5; ++v) o[v] -= a.bgk.r[9 + v];
-            } else {
-                outlet_eq<R>(x, o);
-            }
-            const size_t mn = (size_t)a.m * a.n;
-#pragma unroll
-            for (int v = 0; v < 5; ++v) a.out_next[v * mn + j + (size_t)a.m * k] = o[v];
-        }
-    }
+    if constexpr (XOPEN) outlet_next<S>(a, w, j, k, x);
 }
 
 // No register target beyond the block size: asking the compiler for 8 wavefronts per SIMD (64
@@ -339,24 +346,7 @@ __device__ __forceinline__ void update_mixed_at_once(const XArgs<S> &a, uint32_t
     if constexpr (kOnStored<S>) collide_dev<Store<S>::deviation>(x, a.bgk);
     else collide<R>(x, x, a.omega, a.fgrad);
     put_all<S>(Q19{}, a, reinterpret_cast<char *>(a.B) + (int64_t)t * TB, w.o0, x);
-    if constexpr (XOPEN) {
-        if (w.i == a.l - 1) {
-            R o[5];
-            if constexpr (kOnStored<S> && Store<S>::deviation) {
-                R xf[19];
-#pragma unroll
-                for (int q = 0; q < 19; ++q) xf[q] = x[q] + a.bgk.r[q];
-                outlet_eq<R>(xf, o);
-#pragma unroll
-                for (int v = 0; v < 5; ++v) o[v] -= a.bgk.r[9 + v];
-            } else {
-                outlet_eq<R>(x, o);
-            }
-            const size_t mn = (size_t)a.m * a.n;
-#pragma unroll
-            for (int v = 0; v < 5; ++v) a.out_next[v * mn + j + (size_t)a.m * k] = o[v];
-        }
-    }
+    if constexpr (XOPEN) outlet_next<S>(a, w, j, k, x);
 }
 
 template <typename S> constexpr bool kMixedAtOnce = sizeof(S) <= 4;
