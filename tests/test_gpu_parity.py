@@ -205,24 +205,52 @@ def test_physics_tile_skip_is_exact(storage, shape, x_open, monkeypatch):
         assert np.array_equal(out[0], o)
 
 
-@pytest.mark.parametrize("storage,tol,shifted", [("f32", 2e-6, False), ("f16", 2e-4, False),
-                                                 ("bf16", 2e-3, False), ("f16", 5e-5, True),
-                                                 ("bf16", 5e-4, True)])
-def test_physics_reduced_storage_close_to_f64(storage, tol, shifted):
+@pytest.mark.parametrize("storage,tol,shifted,x_open", [("f32", 2e-6, False, True),
+                                                        ("f16", 2e-4, False, True),
+                                                        ("bf16", 2e-3, False, True),
+                                                        ("f16", 5e-5, True, True),
+                                                        ("bf16", 5e-4, True, True),
+                                                        ("f32", 2e-6, False, False),
+                                                        ("f16", 2e-4, False, False)])
+def test_physics_reduced_storage_close_to_f64(storage, tol, shifted, x_open):
     """Populations after 40 steps within ``tol`` (absolute; populations are O(1/36)) of FP64;
-    the free-stream-shifted deviations have the tighter bound."""
+    the free-stream-shifted deviations have the tighter bound.  Open and periodic x."""
     l, m, n = 96, 32, 32
-    announce(f"physics {storage} shifted={shifted} vs oracle f64, cylinder {l}x{m}x{n}, 40 steps")
+    announce(f"physics {storage} shifted={shifted} x_open={x_open} vs oracle f64, cylinder "
+             f"{l}x{m}x{n}, 40 steps")
     mask, pops = physics_case("cylinder", l, m, n, seed=5)
     svisc = (2 * 0.6 - 1) / 6
     ref = pops.copy()
-    orc.physics_steps(ref, mask, orc.omega_from_svisc(svisc), 0.05, 40)
+    orc.physics_steps(ref, mask, orc.omega_from_svisc(svisc), 0.05, 40, x_open=x_open)
     with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="physics", storage=storage,
-                 shifted=shifted) as lat:
+                 shifted=shifted, x_open=x_open) as lat:
         lat.set_mask(mask)
         lat.set_state(pops)
         lat.step(40)
         got = lat.get_state()
+    fluid = mask == 0
+    err = np.abs(interior(got) - interior(ref))[:, fluid].max()
+    assert err < tol, f"{storage}: max abs population error {err}"
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 2e-6), ("f16", 2e-4), ("bf16", 2e-3)])
+@pytest.mark.parametrize("x_open", [True, False])
+def test_physics_reduced_storage_scattered_solids(storage, tol, x_open):
+    """Scattered solid sites (every tile mixed, lanes with many solid neighbours): the reduced
+    storages' mixed-tile path (all loads at once, bounce-back chosen per lane) against FP64."""
+    l, m, n = 130, 6, 5
+    announce(f"physics {storage} x_open={x_open} scattered solids {l}x{m}x{n} vs oracle f64")
+    mask, pops = physics_case("random", l, m, n, seed=8)
+    svisc = (2 * 0.6 - 1) / 6
+    ref = pops.copy()
+    orc.physics_steps(ref, mask, orc.omega_from_svisc(svisc), 0.05, 40, x_open=x_open)
+    with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="physics", storage=storage,
+                 x_open=x_open) as lat:
+        lat.set_mask(mask)
+        lat.set_state(pops)
+        lat.step(40)
+        got = lat.get_state()
+        assert lat.info()["tiles_mixed"] > 0
     fluid = mask == 0
     err = np.abs(interior(got) - interior(ref))[:, fluid].max()
     assert err < tol, f"{storage}: max abs population error {err}"
