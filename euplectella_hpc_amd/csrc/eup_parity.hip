@@ -37,6 +37,7 @@ struct PArgs {
     double omega, fgrad, u_in;
     double *outp;     // outlet equilibria of the x = l sites, 5 x m x n (written by the stream-collide)
     const uint8_t *solid;
+    UDiv dpitch, dmp; // / pitch and / (m + 2): ghosted offset -> (i, j, k)
 };
 
 __device__ __forceinline__ int64_t gidx(int lp, int mp, int i, int j, int k)
@@ -82,15 +83,18 @@ __global__ void k_bb_gather(PArgs a, const int32_t *list, int64_t nsolid, const 
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsolid) return;
-    const int lp = a.pitch, mp = a.m + 2;
     const int64_t s = list[t];
-    const int i = (int)(s % lp);
-    const int j = (int)((s / lp) % mp);
-    const int k = (int)(s / ((int64_t)lp * mp));
+    // offsets of the padded ghosted layout stay below 2^31
+    const uint32_t rowi = a.dpitch.div((uint32_t)s);
+    const uint32_t kk = a.dmp.div(rowi);
+    const int i = (int)((uint32_t)s - rowi * (uint32_t)a.pitch);
+    const int j = (int)(rowi - kk * (uint32_t)(a.m + 2));
+    const int k = (int)kk;
     double out[18];
     bb_all(Q18{}, a, solid, i, j, k, s, out);
+    // population-major, so that consecutive solid sites write consecutive values
 #pragma unroll
-    for (int q = 0; q < 18; ++q) tmp[t * 18 + q] = out[q];
+    for (int q = 0; q < 18; ++q) tmp[q * nsolid + t] = out[q];
 }
 
 __global__ void k_bb_scatter(PArgs a, const int32_t *list, int64_t nsolid, const double *tmp)
@@ -99,7 +103,7 @@ __global__ void k_bb_scatter(PArgs a, const int32_t *list, int64_t nsolid, const
     if (t >= nsolid) return;
     const int64_t s = list[t];
 #pragma unroll
-    for (int q = 0; q < 18; ++q) const_cast<double *>(a.A[q])[s] = tmp[t * 18 + q];
+    for (int q = 0; q < 18; ++q) const_cast<double *>(a.A[q])[s] = tmp[q * nsolid + t];
 }
 
 __device__ __forceinline__ void put2(const PArgs &a, int q, int64_t s, double v)
@@ -296,6 +300,8 @@ PArgs make_args(Handle *h, int from)
     a.u_in = h->cfg.u_inflow;
     a.outp = static_cast<double *>(h->outlet[0]);
     a.solid = h->solid;
+    a.dpitch = UDiv::make((uint32_t)h->ppitch);
+    a.dmp = UDiv::make((uint32_t)(h->g.m + 2));
     return a;
 }
 
