@@ -10,10 +10,10 @@ module eup
 
    integer(c_int32_t), parameter :: EUP_STORE_F64 = 0, EUP_STORE_F32 = 1
    integer(c_int32_t), parameter :: EUP_STORE_F16 = 2, EUP_STORE_BF16 = 3
-   integer(c_int32_t), parameter :: EUP_MODE_PARITY = 0, EUP_MODE_PHYSICS = 1
+   integer(c_int32_t), parameter :: EUP_MODE_PARITY = 0, EUP_MODE_PHYSICS = 1, EUP_MODE_GREY = 2
    integer(c_int32_t), parameter :: EUP_FACE_PERIODIC = 0, EUP_FACE_INFLOW = 4
    integer(c_int32_t), parameter :: EUP_FACE_OUTFLOW = 5
-   integer(c_int32_t), parameter :: EUP_FLAG_NO_TILE_SKIP = 1
+   integer(c_int32_t), parameter :: EUP_FLAG_NO_TILE_SKIP = 1, EUP_FLAG_SHIFTED_DEVIATION = 2
    integer(c_int32_t), parameter :: EUP_VTK_YZ = 0, EUP_VTK_XZ = 1, EUP_VTK_XY = 2, EUP_VTK_3D = 3
 
    ! mirrors struct eup_config (flags is uint32_t in C; same size and alignment)
@@ -25,6 +25,15 @@ module eup
       integer(c_int32_t) :: flags
       real(c_double) :: omega, fgrad, u_inflow
    end type eup_config
+
+   ! mirrors struct eup_info
+   type, bind(c) :: eup_info
+      integer(c_int64_t) :: sites, nsolid
+      integer(c_int64_t) :: tiles, tiles_fluid, tiles_mixed, tiles_solid
+      integer(c_int32_t) :: bbox(6)
+      integer(c_int32_t) :: bytes_per_site
+      integer(c_int64_t) :: device_bytes
+   end type eup_info
 
    interface
       integer(c_int) function eup_create(cfg, h) bind(c, name='eup_create')
@@ -131,6 +140,97 @@ module eup
       integer(c_int32_t) function eup_abi_version() bind(c, name='eup_abi_version')
          import :: c_int32_t
       end function eup_abi_version
+
+      integer(c_int) function eup_set_state(h, a) bind(c, name='eup_set_state')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: a(*)
+      end function eup_set_state
+
+      integer(c_int) function eup_get_state(h, a) bind(c, name='eup_get_state')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(out) :: a(*)
+      end function eup_get_state
+
+      integer(c_int) function eup_init_equilibrium(h, ux, uy, uz, noise, seed) &
+            bind(c, name='eup_init_equilibrium')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: h
+         real(c_double), value :: ux, uy, uz, noise
+         integer(c_int64_t), value :: seed
+      end function eup_init_equilibrium
+
+      integer(c_int) function eup_set_omega(h, omega) bind(c, name='eup_set_omega')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), value :: omega
+      end function eup_set_omega
+
+      integer(c_int) function eup_sync(h) bind(c, name='eup_sync')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function eup_sync
+
+      integer(c_int) function eup_get_info(h, info) bind(c, name='eup_get_info')
+         import :: c_int, c_ptr, eup_info
+         type(c_ptr), value :: h
+         type(eup_info), intent(out) :: info
+      end function eup_get_info
+
+      type(c_ptr) function eup_stream(h) bind(c, name='eup_stream')
+         import :: c_ptr
+         type(c_ptr), value :: h
+      end function eup_stream
+
+      integer(c_int) function eup_membw(device, bytes, reps, out) bind(c, name='eup_membw')
+         import :: c_int, c_int32_t, c_int64_t, c_double
+         integer(c_int32_t), value :: device
+         integer(c_int64_t), value :: bytes
+         integer(c_int32_t), value :: reps
+         real(c_double), intent(out) :: out(3)
+      end function eup_membw
+
+      ! grey lattice and adjoint (EUP_MODE_GREY)
+      integer(c_int) function eup_set_grey(h, s) bind(c, name='eup_set_grey')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: s(*)
+      end function eup_set_grey
+
+      integer(c_int) function eup_set_grey_device(h, s, dtype) bind(c, name='eup_set_grey_device')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h, s
+         integer(c_int32_t), value :: dtype
+      end function eup_set_grey_device
+
+      integer(c_int) function eup_grey_force(h, f) bind(c, name='eup_grey_force')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(out) :: f(3)
+      end function eup_grey_force
+
+      ! grad may be c_null_ptr (objective only) or c_loc of an l*m*n real(c_double) array
+      integer(c_int) function eup_grey_objective(h, t0, t1, every, norm, j, grad) &
+            bind(c, name='eup_grey_objective')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: t0, t1, every
+         real(c_double), value :: norm
+         real(c_double), intent(out) :: j
+         type(c_ptr), value :: grad
+      end function eup_grey_objective
+
+      integer(c_int) function eup_grey_objective_device(h, t0, t1, every, norm, j, grad, dtype) &
+            bind(c, name='eup_grey_objective_device')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: t0, t1, every
+         real(c_double), value :: norm
+         real(c_double), intent(out) :: j
+         type(c_ptr), value :: grad
+         integer(c_int32_t), value :: dtype
+      end function eup_grey_objective_device
    end interface
 
 contains

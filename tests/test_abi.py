@@ -67,3 +67,11 @@ def test_lattice_requires_a_device_on_cpu_hosts():
     with pytest.raises(eup.EupError) as ei:
         eup.Lattice(8, 8, 8, svisc=0.1, u_inflow=0.05)
     assert "EUP_EHIP" in str(ei.value)
+
+
+def test_fortran_module_declares_every_entry_point():
+    """euplectella_hpc_amd/fortran/eup_mod.F90 binds every function of include/eup.h."""
+    with open(os.path.join(ROOT, "euplectella_hpc_amd", "fortran", "eup_mod.F90")) as fh:
+        text = fh.read()
+    bound = sorted(set(re.findall(r"bind\(c,\s*name='(eup_[a-z_]+)'\)", text)))
+    assert bound == header_functions()
