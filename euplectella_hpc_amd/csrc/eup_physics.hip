@@ -289,9 +289,6 @@ __device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, const
     if constexpr (XOPEN) outlet_next<S>(a, w, j, k, x);
 }
 
-// No register target beyond the block size: asking the compiler for 8 wavefronts per SIMD (64
-// registers) makes it interleave the 19 loads with their uses and costs the 2-byte storages
-// 7-13 % on MI355X, although it removes only one register.
 // Mixed tile with every load issued at once: the link masks, the 19 pulls from the fluid
 // addresses and the site's own 18 moving populations (the bounce-back sources), then a per-lane
 // choice in registers.  update_tile<MIXED> instead waits for the link masks before it can address
@@ -351,6 +348,9 @@ __device__ __forceinline__ void update_mixed_at_once(const XArgs<S> &a, uint32_t
 
 template <typename S> constexpr bool kMixedAtOnce = sizeof(S) <= 4;
 
+// No register target beyond the block size: asking the compiler for 8 wavefronts per SIMD (64
+// registers) makes it interleave the 19 loads with their uses and costs the 2-byte storages
+// 7-13 % on MI355X, although it removes only one register.
 template <typename S, bool XOPEN>
 __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
 {
