@@ -153,6 +153,7 @@ def interior(p):
                                                 ("block", True, (200, 24, 20)),
                                                 ("block", False, (192, 24, 20)),
                                                 ("random", True, (5, 3, 2)),
+                                                ("random", False, (7, 3, 2)),
                                                 ("random", False, (67, 2, 3)),
                                                 ("random", True, (130, 5, 4))])
 def test_physics_f64_equals_oracle(kind, x_open, shape, remap, monkeypatch):
@@ -234,11 +235,13 @@ def test_physics_reduced_storage_close_to_f64(storage, tol, shifted, x_open):
 
 
 @pytest.mark.parametrize("storage,tol", [("f32", 2e-6), ("f16", 2e-4), ("bf16", 2e-3)])
-@pytest.mark.parametrize("x_open", [True, False])
-def test_physics_reduced_storage_scattered_solids(storage, tol, x_open):
+@pytest.mark.parametrize("x_open,shape", [(True, (130, 6, 5)), (False, (130, 6, 5)),
+                                          (False, (7, 3, 2))])
+def test_physics_reduced_storage_scattered_solids(storage, tol, x_open, shape):
     """Scattered solid sites (every tile mixed, lanes with many solid neighbours): the reduced
-    storages' mixed-tile path (all loads at once, bounce-back chosen per lane) against FP64."""
-    l, m, n = 130, 6, 5
+    storages' mixed-tile path (all loads at once, bounce-back chosen per lane) against FP64;
+    also a lattice narrower than one tile with periodic x (first and last tile the same)."""
+    l, m, n = shape
     announce(f"physics {storage} x_open={x_open} scattered solids {l}x{m}x{n} vs oracle f64")
     mask, pops = physics_case("random", l, m, n, seed=8)
     svisc = (2 * 0.6 - 1) / 6
