@@ -18,8 +18,11 @@
 //   lam_p(z) = (1 - sigma_l) lin_p(z + c_p) + sigma_l lin_pbar(z)   (sigma_l of link z -- z + c_p)
 // plus the outlet-equilibrium transpose on the last x column and the objective terms, and adds the
 // gradient terms of the links of which the site is the source (FP64 accumulation).
+// Inside a recomputed segment k_adj_fused runs the stream half of step t and the collide half of
+// step t - 1 in one pass, so lam^t is never stored.
 // States are checkpointed every K steps; each segment is recomputed forward and stored before its
 // backward sweep, so memory is (t1/K + K) states instead of t1.
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -276,8 +279,136 @@ template <typename R> struct AdjArgs {
     double wobj;     // objective weight at time t (0 outside the window)
 };
 
-// lin(x) = J_C(f(x))^T lam^{t+1}(x), and the gradient terms of the links that x pulls through
-// (x as the pulling site: d f_q(x) / d s(x) on every link, plus the objective's force term)
+// lin(x) = J_C(f(x))^T lam^{t+1}(x) from lam[19] = lam^{t+1}(x) (overwritten), stored to a.lin;
+// g gets the gradient terms of the links that x pulls through (x as the pulling site:
+// d f_q(x) / d s(x) on every link, plus the objective's force term)
+template <typename R>
+__device__ __forceinline__ void adj_collide_site(const AdjArgs<R> &a, const Nbr &nb, int i, int j, int k, int64_t x,
+                                                 R *lam, double &g)
+{
+    const int64_t st = a.g.stride;
+    // FP32 keeps each link's source value, source fraction and kind from the pull for the
+    // gradient terms; FP64 reads them again (cache hits) instead, which lowers its registers from
+    // 306 to 248 per lane (one to two wavefronts per SIMD).  Measured on MI355X (128 x 64 x 64):
+    // FP64 117.5 -> 112.6 us, while the FP32 kernel would go from 42 to 51-62 us.
+    const double w = a.wobj;
+    if constexpr (sizeof(R) == 4) {
+        R f[19], sig[18], src[18], sy[18], out[19];
+        int kind[18];
+        grey_pull_links<R>(a.g, nb, i, j, k, x, f, sig, src, sy, kind);
+        collide_adj<R>(f, lam, a.g.omega, out);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) a.lin[q * st + x] = out[q];
+#pragma unroll
+        for (int q = 0; q < 18; ++q) {
+            const double L = (double)out[q];
+            const double abar = (double)a.g.a(kOPP[q], x);
+            const double syq = (double)sy[q];
+            g += L * (abar - (double)src[q]) * (1.0 - syq);
+            if (kind[q] == 0 && w != 0.0) g += w * kCX[q] * ((double)src[q] - abar) * (1.0 - syq);
+        }
+    } else {
+        R f[19];
+        grey_pull<R>(a.g, nb, i, j, k, x, f);
+        collide_adj<R>(f, lam, a.g.omega, lam); // in place: out[p] reads only lam[p] after the sums
+#pragma unroll
+        for (int q = 0; q < 19; ++q) a.lin[q * st + x] = lam[q];
+#pragma unroll
+        for (int q = 0; q < 18; ++q) {
+            R sv, syv;
+            int kd;
+            grey_link<R>(a.g, nb, i, j, k, q, sv, syv, kd);
+            const double L = (double)lam[q];
+            const double abar = (double)a.g.a(kOPP[q], x);
+            const double syq = (double)syv;
+            g += L * (abar - (double)sv) * (1.0 - syq);
+            if (kd == 0 && w != 0.0) g += w * kCX[q] * ((double)sv - abar) * (1.0 - syq);
+        }
+    }
+}
+
+// lam^t(z) into lam[19]: the transposed pull gathered from a.lin at the sites z + c_p that pull
+// from z, the outlet-equilibrium transpose on the last x column and the objective's force term;
+// g gets the gradient terms of the links of which z is the source
+template <typename R>
+__device__ __forceinline__ void adj_stream_site(const AdjArgs<R> &a, const Nbr &nb, int i, int j, int k, int64_t z,
+                                                R *lam, double &g)
+{
+    const int l = a.g.l;
+    const int64_t st = a.g.stride;
+    const double w = a.wobj;
+    const R sz = a.g.s[z];
+#pragma unroll
+    for (int p = 0; p < 18; ++p) {
+        const int pb = kOPP[p];
+        const int64_t zp = nb.at(i, j, k, kCX[p], kCY[p], kCZ[p]); // the site pulling p from z
+        double lp;
+        if (zp >= 0) {
+            const R s2 = a.g.s[zp];
+            const double sl = 1.0 - (1.0 - (double)sz) * (1.0 - (double)s2);
+            const double Lp = (double)a.lin[p * st + zp];
+            lp = (1.0 - sl) * Lp + sl * (double)a.lin[pb * st + z];
+            if (w != 0.0) lp += 2.0 * w * sl * kCX[p];
+            const double d = (double)a.g.a(pb, zp) - (double)a.g.a(p, z);
+            g += Lp * d * (1.0 - (double)s2);
+            if (w != 0.0) g -= w * kCX[p] * d * (1.0 - (double)s2);
+        } else {
+            lp = (double)sz * (double)a.lin[pb * st + z]; // ghost link: sigma = s(z)
+        }
+        lam[p] = (R)lp;
+    }
+    lam[18] = a.lin[18 * st + z];
+    // outlet equilibrium transpose: z on the last x column feeds the ghost values read by the
+    // sites x = (l-1, j + cy, k + cz) for q = 10..14 (0-based 9..13)
+    if (a.g.xopen && i == l - 1) {
+        double Jx = 0, Jy = 0, Jz = 0;
+#pragma unroll
+        for (int p = 0; p < 19; ++p) {
+            const double yp = (double)a.g.a(p, z);
+            Jx += kCX[p] * yp;
+            Jy += kCY[p] * yp;
+            Jz += kCZ[p] * yp;
+        }
+        double Lw[5], cJ[5];
+#pragma unroll
+        for (int q = 9; q < 14; ++q) {
+            const int64_t x = nb.at(l - 1, j, k, 0, kCY[q], kCZ[q]);
+            const double sg = (double)a.g.s[x]; // sigma of the ghost link
+            const double L = (1.0 - sg) * (double)a.lin[q * st + x];
+            Lw[q - 9] = L * (double)wq<double>(q);
+            cJ[q - 9] = kCX[q] * Jx + kCY[q] * Jy + kCZ[q] * Jz;
+        }
+        // per population p the five terms in the order q = 10..14, one p at a time
+#pragma unroll
+        for (int p = 0; p < 19; ++p) {
+            const double Jc = Jx * kCX[p] + Jy * kCY[p] + Jz * kCZ[p];
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 9; q < 14; ++q) {
+                const double cc = kCX[q] * kCX[p] + kCY[q] * kCY[p] + kCZ[q] * kCZ[p];
+                acc += Lw[q - 9] * (3.0 * cc + 9.0 * cJ[q - 9] * cc - 3.0 * Jc);
+            }
+            lam[p] = (R)((double)lam[p] + acc);
+        }
+    }
+}
+
+// one thread per site: (i, j, k) of site x, false past the lattice
+__device__ __forceinline__ bool site_of(int l, int m, int n, int64_t &x, int &i, int &j, int &k)
+{
+    x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= (int64_t)l * m * n) return false;
+    i = (int)(x % l);
+    const int64_t r = x / l;
+    j = (int)(r % m);
+    k = (int)(r / m);
+    return true;
+}
+
+// the collide half of an adjoint step on its own: lin = J_C^T lam_in, and the gradient terms of
+// the links that x pulls through.  The same arithmetic as adj_collide_site written out with lam
+// read after the pull: built on adj_collide_site the FP32 kernel takes 170 registers per lane
+// instead of 166 (two wavefronts per SIMD instead of three; 47 against 42 us on 128 x 64 x 64).
 template <typename R> __global__ __launch_bounds__(256) void k_adj_collide(AdjArgs<R> a)
 {
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,10 +419,6 @@ template <typename R> __global__ __launch_bounds__(256) void k_adj_collide(AdjAr
     const int j = (int)(r % a.g.m), k = (int)(r / a.g.m);
     const int64_t st = a.g.stride;
     Nbr nb{a.g.l, a.g.m, a.g.n, a.g.xopen};
-    // FP32 keeps each link's source value, source fraction and kind from the pull for the
-    // gradient terms; FP64 reads them again (cache hits) instead, which lowers its registers from
-    // 306 to 248 per lane (one to two wavefronts per SIMD).  Measured on MI355X (128 x 64 x 64):
-    // FP64 117.5 -> 112.6 us, while the FP32 kernel would go from 42 to 51-62 us.
     const double w = a.wobj;
     double g = 0.0;
     if constexpr (sizeof(R) == 4) {
@@ -334,77 +461,41 @@ template <typename R> __global__ __launch_bounds__(256) void k_adj_collide(AdjAr
     a.gs[x] += g;
 }
 
-// lam^t(z): the transposed pull gathered from the sites z + c_p that pull from z, the
-// outlet-equilibrium transpose on the last x column and the objective's force term; and the
-// gradient terms of the links of which z is the source
+// the stream half of an adjoint step on its own: lam_out = P^T lin (+ objective terms)
 template <typename R> __global__ __launch_bounds__(256) void k_adj_stream(AdjArgs<R> a)
 {
-    const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = a.g.l, m = a.g.m, n = a.g.n;
-    const int64_t N = (int64_t)l * m * n;
-    if (z >= N) return;
-    const int i = (int)(z % l);
-    const int64_t r = z / l;
-    const int j = (int)(r % m), k = (int)(r / m);
+    int64_t z;
+    int i, j, k;
+    if (!site_of(a.g.l, a.g.m, a.g.n, z, i, j, k)) return;
     const int64_t st = a.g.stride;
-    Nbr nb{l, m, n, a.g.xopen};
-    const double w = a.wobj;
-    const R sz = a.g.s[z];
+    Nbr nb{a.g.l, a.g.m, a.g.n, a.g.xopen};
+    R lam[19];
     double g = 0.0;
+    adj_stream_site<R>(a, nb, i, j, k, z, lam, g);
 #pragma unroll
-    for (int p = 0; p < 18; ++p) {
-        const int pb = kOPP[p];
-        const int64_t zp = nb.at(i, j, k, kCX[p], kCY[p], kCZ[p]); // the site pulling p from z
-        double lam;
-        if (zp >= 0) {
-            const R s2 = a.g.s[zp];
-            const double sl = 1.0 - (1.0 - (double)sz) * (1.0 - (double)s2);
-            const double Lp = (double)a.lin[p * st + zp];
-            lam = (1.0 - sl) * Lp + sl * (double)a.lin[pb * st + z];
-            if (w != 0.0) lam += 2.0 * w * sl * kCX[p];
-            const double d = (double)a.g.a(pb, zp) - (double)a.g.a(p, z);
-            g += Lp * d * (1.0 - (double)s2);
-            if (w != 0.0) g -= w * kCX[p] * d * (1.0 - (double)s2);
-        } else {
-            lam = (double)sz * (double)a.lin[pb * st + z]; // ghost link: sigma = s(z)
-        }
-        a.lam_out[p * st + z] = (R)lam;
-    }
-    a.lam_out[18 * st + z] = a.lin[18 * st + z];
-    // outlet equilibrium transpose: z on the last x column feeds the ghost values read by the
-    // sites x = (l-1, j + cy, k + cz) for q = 10..14 (0-based 9..13)
-    if (a.g.xopen && i == l - 1) {
-        double Jx = 0, Jy = 0, Jz = 0;
-#pragma unroll
-        for (int p = 0; p < 19; ++p) {
-            const double yp = (double)a.g.a(p, z);
-            Jx += kCX[p] * yp;
-            Jy += kCY[p] * yp;
-            Jz += kCZ[p] * yp;
-        }
-        double Lw[5], cJ[5];
-#pragma unroll
-        for (int q = 9; q < 14; ++q) {
-            const int64_t x = nb.at(l - 1, j, k, 0, kCY[q], kCZ[q]);
-            const double sg = (double)a.g.s[x]; // sigma of the ghost link
-            const double L = (1.0 - sg) * (double)a.lin[q * st + x];
-            Lw[q - 9] = L * (double)wq<double>(q);
-            cJ[q - 9] = kCX[q] * Jx + kCY[q] * Jy + kCZ[q] * Jz;
-        }
-        // per population p the five terms in the order q = 10..14, one p at a time
-#pragma unroll
-        for (int p = 0; p < 19; ++p) {
-            const double Jc = Jx * kCX[p] + Jy * kCY[p] + Jz * kCZ[p];
-            double acc = 0.0;
-#pragma unroll
-            for (int q = 9; q < 14; ++q) {
-                const double cc = kCX[q] * kCX[p] + kCY[q] * kCY[p] + kCZ[q] * kCZ[p];
-                acc += Lw[q - 9] * (3.0 * cc + 9.0 * cJ[q - 9] * cc - 3.0 * Jc);
-            }
-            a.lam_out[p * st + z] = (R)((double)a.lam_out[p * st + z] + acc);
-        }
-    }
+    for (int p = 0; p < 19; ++p) a.lam_out[p * st + z] = lam[p];
     a.gs[z] += g;
+}
+
+// the stream half of step t (sa: state a^t, lin^t in sa.lin) and the collide half of step t - 1
+// (ca: state a^{t-1}, lin^{t-1} into ca.lin) in one pass: lam^t stays in registers, so the
+// adjoint field is read and written once per step instead of twice.  The gradient sum of each
+// site takes the two halves' terms in the same order as the split kernels, so the result is
+// bit-identical to running k_adj_stream(t) and k_adj_collide(t - 1).
+template <typename R> __global__ __launch_bounds__(256) void k_adj_fused(AdjArgs<R> sa, AdjArgs<R> ca)
+{
+    int64_t z;
+    int i, j, k;
+    if (!site_of(sa.g.l, sa.g.m, sa.g.n, z, i, j, k)) return;
+    Nbr nb{sa.g.l, sa.g.m, sa.g.n, sa.g.xopen};
+    R lam[19];
+    double g1 = 0.0, g2 = 0.0;
+    adj_stream_site<R>(sa, nb, i, j, k, z, lam, g1);
+    adj_collide_site<R>(ca, nb, i, j, k, z, lam, g2);
+    double v = sa.gs[z];
+    v += g1;
+    v += g2;
+    sa.gs[z] = v;
 }
 
 // objective seed at t1: lam = w dF/da, gs += w dF/ds
@@ -553,20 +644,20 @@ int objective_t(Handle *h, int t0, int t1, int K, double norm, double *J, void *
     *J = sum[0] * wobj;
     if (want_grad) {
         char *seg = static_cast<char *>(alloc(S * (size_t)K));
-        void *lamb[2] = {alloc(S), alloc(S)};
-        void *lin = alloc(S);
+        void *lam = alloc(S);                    // lam^t between the halves of a step
+        void *linb[2] = {alloc(S), alloc(S)};    // lin^t, lin^{t-1}
         double *gs = static_cast<double *>(alloc(sizeof(double) * N));
-        if (!seg || !lamb[0] || !lamb[1] || !lin || !gs) {
+        if (!seg || !lam || !linb[0] || !linb[1] || !gs) {
             cleanup();
             return fail(h, EUP_EHIP, "eup_grey_objective: out of device memory for the adjoint");
         }
         hipMemsetAsync(gs, 0, sizeof(double) * N, h->stream);
-        auto adj = [&](const void *state, int li, int lo, double w) {
+        auto adj = [&](const void *state, const void *lam_in, void *lin, void *lam_out, double w) {
             AdjArgs<R> q;
             q.g = gargs<R>(h, state, nullptr);
-            q.lam_in = static_cast<const R *>(lamb[li]);
+            q.lam_in = static_cast<const R *>(lam_in);
             q.lin = static_cast<R *>(lin);
-            q.lam_out = static_cast<R *>(lamb[lo]);
+            q.lam_out = static_cast<R *>(lam_out);
             q.gs = gs;
             q.wobj = w;
             return q;
@@ -581,23 +672,46 @@ int objective_t(Handle *h, int t0, int t1, int K, double norm, double *J, void *
                 hipLaunchKernelGGL((k_grey_step<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, a);
                 w ^= 1;
             }
-            AdjArgs<R> q = adj(work[w], 1, 0, wobj);
+            AdjArgs<R> q = adj(work[w], nullptr, nullptr, lam, wobj);
             hipLaunchKernelGGL((k_force_adj_seed<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, q);
         }
-        int li = 0; // lamb[li] holds lam^{t+1}
+        // Per segment: the collide half of its last step, then the stream half of step t fused
+        // with the collide half of step t - 1 (both states are in the recomputed segment), then
+        // the stream half of its first step.  Fused by default in FP64 (MI355X, 128 x 64 x 64:
+        // 154 us against 102 + 71 us split); FP32 runs the halves on their own by default (fused
+        // 97 us against 47 + 44 us: the fused kernel's 173 registers per lane allow two
+        // wavefronts per SIMD, the split collide's three).  EUP_ADJ_FUSED=0 / 1 forces either
+        // form; both give the same results bit for bit.
+        bool fuse = sizeof(R) == 8;
+        if (const char *e = getenv("EUP_ADJ_FUSED")) fuse = atoi(e) != 0;
+        const dim3 grid(grid_sites<R>(h));
         for (int cc = (t1 - 1) / K; cc >= 0; --cc) {
             const int tb = cc * K, te = (cc + 1) * K < t1 ? (cc + 1) * K : t1;
             hipMemcpyAsync(seg, ck + S * (size_t)cc, S, hipMemcpyDeviceToDevice, h->stream);
             for (int t = tb + 1; t < te; ++t) {
                 GArgs<R> a = gargs<R>(h, seg + S * (size_t)(t - 1 - tb), seg + S * (size_t)(t - tb));
-                hipLaunchKernelGGL((k_grey_step<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, a);
+                hipLaunchKernelGGL((k_grey_step<R>), grid, dim3(256), 0, h->stream, a);
             }
-            for (int t = te - 1; t >= tb; --t) {
-                AdjArgs<R> q = adj(seg + S * (size_t)(t - tb), li, li ^ 1, t >= t0 ? wobj : 0.0);
-                hipLaunchKernelGGL((k_adj_collide<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, q);
-                hipLaunchKernelGGL((k_adj_stream<R>), dim3(grid_sites<R>(h)), dim3(256), 0, h->stream, q);
-                li ^= 1;
+            auto at = [&](int t) { return static_cast<const void *>(seg + S * (size_t)(t - tb)); };
+            auto wt = [&](int t) { return t >= t0 ? wobj : 0.0; };
+            int b = 0; // linb[b] holds lin^t
+            hipLaunchKernelGGL((k_adj_collide<R>), grid, dim3(256), 0, h->stream,
+                               adj(at(te - 1), lam, linb[b], nullptr, wt(te - 1)));
+            for (int t = te - 1; t > tb; --t) {
+                if (fuse) {
+                    hipLaunchKernelGGL((k_adj_fused<R>), grid, dim3(256), 0, h->stream,
+                                       adj(at(t), nullptr, linb[b], nullptr, wt(t)),
+                                       adj(at(t - 1), nullptr, linb[b ^ 1], nullptr, wt(t - 1)));
+                    b ^= 1;
+                } else {
+                    hipLaunchKernelGGL((k_adj_stream<R>), grid, dim3(256), 0, h->stream,
+                                       adj(at(t), nullptr, linb[b], lam, wt(t)));
+                    hipLaunchKernelGGL((k_adj_collide<R>), grid, dim3(256), 0, h->stream,
+                                       adj(at(t - 1), lam, linb[b], nullptr, wt(t - 1)));
+                }
             }
+            hipLaunchKernelGGL((k_adj_stream<R>), grid, dim3(256), 0, h->stream,
+                               adj(at(tb), nullptr, linb[b], lam, wt(tb)));
         }
         if (hip_check(h, hipGetLastError(), "adjoint sweep")) {
             cleanup();

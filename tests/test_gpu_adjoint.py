@@ -4,6 +4,7 @@
 * objective and gradient (FP64) match the oracle's adjoint;
 * the FP32 adjoint matches the FP64 adjoint, and the FP64 adjoint matches central finite
   differences of the FP64 objective on 64 cells (relative 1e-4 is the stated tolerance);
+* the fused adjoint pass equals the split kernels bit for bit;
 * the torch.autograd.Function returns the same gradient.
 """
 import numpy as np
@@ -89,6 +90,30 @@ def test_objective_and_gradient_f64_match_oracle():
         assert np.array_equal(dense(lat.get_state()), dense(pops))
     assert abs(J - Jr) <= 1e-12 * abs(Jr)
     assert np.abs(g - gr).max() <= 1e-10 * np.abs(gr).max()
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("x_open", [True, False])
+def test_fused_adjoint_sweep_equals_split_kernels(storage, x_open, monkeypatch):
+    """The fused stream + collide adjoint pass gives the split kernels' objective and gradient bit
+    for bit, for segments of one step, several steps with a partial last one, and one segment."""
+    l, m, n = 24, 12, 10
+    s, pops = grey_inputs(l, m, n, seed=5, n_spheres=3)
+    svisc = (2 * 0.6 - 1) / 6
+    t0, t1 = 10, 30
+    for K in (1, 7, 31):
+        announce(f"grey {storage} x_open={x_open} adjoint fused vs split, K={K}")
+        out = {}
+        for fused in ("0", "1"):
+            monkeypatch.setenv("EUP_ADJ_FUSED", fused)
+            with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="grey", storage=storage,
+                         x_open=x_open) as lat:
+                lat.set_grey(s)
+                lat.set_state(pops)
+                out[fused] = lat.grey_objective(t0, t1, K, 1.0)
+        assert out["0"][0] == out["1"][0]
+        assert np.array_equal(out["0"][1].view(np.uint64), out["1"][1].view(np.uint64))
+        assert np.abs(out["1"][1]).max() > 0
 
 
 C4 = dict(l=128, m=64, n=64, tau=0.6, u_in=0.05, t0=1000, t1=2000, every=100, seed=44)
