@@ -36,21 +36,23 @@ def grey_inputs(l, m, n, seed, n_spheres=4, rmin=2.0, rmax=4.0, sigma=1.0):
     return s, pops
 
 
-def test_grey_f64_step_equals_oracle():
+@pytest.mark.parametrize("x_open", [True, False])
+def test_grey_f64_step_equals_oracle(x_open):
     l, m, n = 40, 20, 16
-    announce(f"grey f64 step {l}x{m}x{n} vs oracle, 20 steps")
+    announce(f"grey f64 step {l}x{m}x{n} x_open={x_open} vs oracle, 20 steps")
     s, pops = grey_inputs(l, m, n, seed=1)
     svisc = (2 * 0.6 - 1) / 6
     ref = dense(pops)
-    orc.grey_steps(ref, s, orc.omega_from_svisc(svisc), 0.05, 20)
-    with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="grey", storage="f64") as lat:
+    orc.grey_steps(ref, s, orc.omega_from_svisc(svisc), 0.05, 20, x_open=x_open)
+    with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="grey", storage="f64",
+                 x_open=x_open) as lat:
         lat.set_grey(s)
         lat.set_state(pops)
         lat.step(20)
         got = dense(lat.get_state())
         F = lat.grey_force()
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
-    Fr = orc.grey_force(ref, s, 0.05)
+    Fr = orc.grey_force(ref, s, 0.05, x_open=x_open)
     assert np.abs(F - Fr).max() <= 1e-12 * np.abs(Fr).max()
 
 
@@ -74,15 +76,17 @@ def test_binary_grey_field_equals_physics_mode_on_fluid_sites():
     assert np.array_equal(out["grey"][:, fluid], out["physics"][:, fluid])
 
 
-def test_objective_and_gradient_f64_match_oracle():
+@pytest.mark.parametrize("x_open", [True, False])
+def test_objective_and_gradient_f64_match_oracle(x_open):
     l, m, n = 24, 12, 10
-    announce(f"grey f64 objective + adjoint {l}x{m}x{n} vs oracle adjoint")
+    announce(f"grey f64 objective + adjoint {l}x{m}x{n} x_open={x_open} vs oracle adjoint")
     s, pops = grey_inputs(l, m, n, seed=3, n_spheres=3)
     svisc = (2 * 0.6 - 1) / 6
     om = orc.omega_from_svisc(svisc)
     t0, t1, K = 10, 30, 7
-    Jr, gr = orc.grey_objective(dense(pops), s, om, 0.05, t0, t1, K, 1.0)
-    with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="grey", storage="f64") as lat:
+    Jr, gr = orc.grey_objective(dense(pops), s, om, 0.05, t0, t1, K, 1.0, x_open=x_open)
+    with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="grey", storage="f64",
+                 x_open=x_open) as lat:
         lat.set_grey(s)
         lat.set_state(pops)
         J, g = lat.grey_objective(t0, t1, K, 1.0)

@@ -4,7 +4,7 @@
   bit on fluid sites; s = 0 gives the plain fluid step;
 * partial bounce-back conserves mass on every link (periodic box);
 * the collision adjoint is the transpose of the collision Jacobian (dot-product test against
-  central differences);
+  central differences); the stream adjoint is the inverse permutation of the pull;
 * the full adjoint gradient of the time-averaged drag matches central finite differences.
 """
 import numpy as np
@@ -83,6 +83,27 @@ def test_step_adjoint_dot_product_against_central_differences():
     lam_out, _ = orc.grey_step_adj(a, s, om, u, lam)
     rhs = np.sum(lam_out * d)
     assert abs(lhs - rhs) < 1e-7 * abs(lhs)
+
+
+def test_stream_adjoint_is_the_inverse_permutation():
+    """With omega = 0 (collision = identity) and s = 0 on a periodic box the step is the pull
+    permutation b_q(x) = a_q(x - c_q), so its adjoint must be the inverse shift lam_q(x + c_q)."""
+    l, m, n = 9, 7, 5
+    rng = np.random.default_rng(7)
+    a = dense(geometry.initial_state(l, m, n, 0.05, noise=1e-3, seed=3))
+    lam = rng.standard_normal(a.shape)
+    lam_out, _ = orc.grey_step_adj(a, np.zeros((n, m, l)), 0.0, 0.05, lam, x_open=False)
+    want = np.empty_like(lam)
+    for q in range(19):
+        cx, cy, cz = (int(v) for v in d3q19.C[q])
+        want[q] = np.roll(lam[q], (-cz, -cy, -cx), axis=(0, 1, 2))
+    assert np.array_equal(lam_out, want)
+    # and the forward step is that permutation
+    b = a.copy()
+    orc.grey_steps(b, np.zeros((n, m, l)), 0.0, 0.05, 1, x_open=False)
+    for q in range(19):
+        cx, cy, cz = (int(v) for v in d3q19.C[q])
+        assert np.array_equal(b[q], np.roll(a[q], (cz, cy, cx), axis=(0, 1, 2)))
 
 
 def grey_case(l=16, m=10, n=8, seed=6):
