@@ -687,6 +687,7 @@ int objective_t(Handle *h, int t0, int t1, int K, double norm, double *J, void *
         const dim3 grid(grid_sites<R>(h));
         for (int cc = (t1 - 1) / K; cc >= 0; --cc) {
             const int tb = cc * K, te = (cc + 1) * K < t1 ? (cc + 1) * K : t1;
+            if (te <= tb) continue; // no step in this segment (t1 >= 1 keeps every segment non-empty)
             hipMemcpyAsync(seg, ck + S * (size_t)cc, S, hipMemcpyDeviceToDevice, h->stream);
             for (int t = tb + 1; t < te; ++t) {
                 GArgs<R> a = gargs<R>(h, seg + S * (size_t)(t - 1 - tb), seg + S * (size_t)(t - tb));
