@@ -96,6 +96,23 @@ def test_objective_and_gradient_f64_match_oracle(x_open):
     assert np.abs(g - gr).max() <= 1e-10 * np.abs(gr).max()
 
 
+def test_objective_windows_at_the_edges_match_oracle():
+    """Windows of one step, checkpoint intervals longer than the run, a window starting at step 1."""
+    l, m, n = 16, 8, 6
+    s, pops = grey_inputs(l, m, n, seed=9, n_spheres=2, rmin=1.5, rmax=2.5)
+    svisc = (2 * 0.6 - 1) / 6
+    om = orc.omega_from_svisc(svisc)
+    for t0, t1, K in [(1, 1, 1), (1, 1, 4), (1, 3, 2), (2, 9, 9), (2, 9, 20)]:
+        announce(f"grey f64 objective + adjoint t0={t0} t1={t1} K={K} vs oracle")
+        Jr, gr = orc.grey_objective(dense(pops), s, om, 0.05, t0, t1, K, 1.0)
+        with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="grey", storage="f64") as lat:
+            lat.set_grey(s)
+            lat.set_state(pops)
+            J, g = lat.grey_objective(t0, t1, K, 1.0)
+        assert abs(J - Jr) <= 1e-12 * abs(Jr)
+        assert np.abs(g - gr).max() <= 1e-10 * np.abs(gr).max()
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32"])
 @pytest.mark.parametrize("x_open", [True, False])
 def test_fused_adjoint_sweep_equals_split_kernels(storage, x_open, monkeypatch):
