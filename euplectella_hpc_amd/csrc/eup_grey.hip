@@ -18,8 +18,8 @@
 //   lam_p(z) = (1 - sigma_l) lin_p(z + c_p) + sigma_l lin_pbar(z)   (sigma_l of link z -- z + c_p)
 // plus the outlet-equilibrium transpose on the last x column and the objective terms, and adds the
 // gradient terms of the links of which the site is the source (FP64 accumulation).
-// Inside a recomputed segment k_adj_fused runs the stream half of step t and the collide half of
-// step t - 1 in one pass, so lam^t is never stored.
+// Inside a recomputed segment k_adj_fused (the FP64 default) runs the stream half of step t and the
+// collide half of step t - 1 in one pass, so lam^t is never stored.
 // States are checkpointed every K steps; each segment is recomputed forward and stored before its
 // backward sweep, so memory is (t1/K + K) states instead of t1.
 #include <cstdlib>
