@@ -62,7 +62,7 @@ struct Handle {
     bool outlet_dirty = true;      // the host wrote populations: physics mode recomputes its outlet
                                    // plane, parity mode refills the constant inflow plane
     size_t fused_lds = (size_t)-1; // dynamic LDS per block of the fused kernel (occupancy cap)
-    int fused_remap = -1;          // fused kernel block order: 1 XCD-contiguous, 0 dispatch order
+    int fused_remap = -1;          // fused kernel block order: 0 dispatch, 1 XCD-contiguous, 2 XCD-owned rows
     size_t parity_lds = (size_t)-1; // the same for the parity stream-collide kernel
     // diagnostics scratch
     double *diag = nullptr;       // device scratch for reductions

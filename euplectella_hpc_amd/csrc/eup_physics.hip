@@ -49,8 +49,10 @@ template <typename S> struct XArgs {
     int l, m, n, tpr;
     UDiv dtpr, dm; // t / tpr and row / m (t < tiles < 2^31)
     uint32_t tiles;
-    uint32_t chunk; // logical blocks per XCD
-    uint32_t remap; // 1: XCD-contiguous block order, 0: dispatch order
+    uint32_t chunk; // logical blocks per XCD (order 1)
+    uint32_t grp;   // logical blocks of one tile row, rounded up (order 2)
+    UDiv dgrp;      // n / grp
+    uint32_t order; // block order: 0 dispatch, 1 XCD-contiguous ranges, 2 XCD-owned tile rows
 };
 
 // FP32-compute storages (FP32, FP16 and BF16 deviations) run the step on the stored values
@@ -354,11 +356,21 @@ template <typename S> constexpr bool kMixedAtOnce = sizeof(S) <= 4;
 template <typename S, bool XOPEN>
 __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
 {
-    // Block order (chosen by fused_block_order).  Hardware block b runs on XCD b % 8.  With remap
-    // each XCD gets a contiguous range of tiles; without it the eight XCDs sweep the lattice
-    // together in dispatch order.
+    // Block order (chosen by fused_block_order).  Hardware block b runs on XCD b % 8.  Order 0:
+    // the eight XCDs sweep the lattice together in dispatch order.  Order 1: each XCD gets a
+    // contiguous range of tiles.  Order 2: the XCDs still sweep together, but every group of
+    // 8 * grp hardware blocks is dealt out so that each XCD takes grp consecutive logical blocks
+    // (a whole tile row), so the x-neighbouring tiles that share cache lines through their
+    // x-shifted pulls run on one XCD and share its L2; a last partial group keeps dispatch order
+    // (the mapping stays a bijection).
     const uint32_t b = blockIdx.x;
-    const uint32_t lb = a.remap ? (b & 7u) * a.chunk + (b >> 3) : b;
+    uint32_t lb = b;
+    if (a.order == 1) {
+        lb = (b & 7u) * a.chunk + (b >> 3);
+    } else if (a.order == 2) {
+        const uint32_t r8 = b >> 3, gq = a.dgrp.div(r8), g0 = gq * 8u * a.grp;
+        if (g0 + 8u * a.grp <= gridDim.x) lb = g0 + (b & 7u) * a.grp + (r8 - gq * a.grp);
+    }
     const uint32_t t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kTile) + (threadIdx.x >> 6));
     if (t >= a.tiles) return;
     // the codes are read-only during the launch: a scalar load through the constant address space
@@ -597,7 +609,9 @@ template <typename S> XArgs<S> make_args(Handle *h)
     const uint32_t per_block = kBlock / kTile;
     const uint32_t nb = (uint32_t)((h->g.tiles + per_block - 1) / per_block);
     a.chunk = (nb + 7) / 8;
-    a.remap = h->fused_remap > 0 ? 1u : 0u; // set by the step launch (fused_block_order)
+    a.grp = (uint32_t)((h->g.tiles_per_row + per_block - 1) / per_block);
+    a.dgrp = UDiv::make(a.grp);
+    a.order = h->fused_remap > 0 ? (uint32_t)h->fused_remap : 0u; // set by the step launch (fused_block_order)
     return a;
 }
 
@@ -624,22 +638,28 @@ static int fused_cap(const Handle *h)
 }
 
 // Block order of the fused kernel.  In dispatch order the resident wavefronts of the whole GPU
-// form one window that moves through the lattice; when that window covers at least two planes of
-// tiles (z = k - 1, k, k + 1 are then resident together), the rows a tile pulls from are fetched
-// while their other readers are resident too, and dispatch order is faster.  Otherwise each XCD
-// is given a contiguous range of tiles.  Measured on MI355X (window / plane of tiles, dispatch
-// order against remap): 512 x 256 x 256 sponge FP32 (2.5) +2.3 %, FP16 (4.0) +2.2 %, FP64 (1.5)
-// -5 %; 1536 x 768 x 768 sponge FP32 (0.28) -7 %, FP16 (0.44) -4 %; its 192 x 768 x 768 slab
-// FP32 (2.2) +8 %; 256 x 128 x 128 FP32 (10) +13 %.  EUP_FUSED_XCD_REMAP=0/1 forces the choice.
+// form one window that moves through the lattice; when that window covers at least a plane of
+// tiles, the rows a tile pulls from are fetched while their other readers are resident too, and
+// the XCDs sweeping together is faster, best with each XCD owning whole tile rows (order 2: the
+// lines that x-neighbouring tiles share through their x-shifted pulls stay in one L2).  Otherwise
+// each XCD is given a contiguous range of tiles (order 1).  Measured on MI355X (window / plane of
+// tiles; MLUPS in orders 0 / 1 / 2): 512 x 256 x 256 sponge FP32 (2.5) 39 100-39 400 / - /
+// 40 000-40 200, FP16 (4.0) 75 100-75 300 / - / 78 000, BF16 (4.0) - / 71 300 / 78 500, FP64 (1.5)
+// - / 18 970 / 20 210; 1536 x 768 x 768 sponge FP32 (0.28) - / 39 280 / 36 360, FP16 (0.44)
+// - / 77 900 / 74 700; its 192 x 768 x 768 slab FP32 (2.2) 39 810 / - / 40 290.
+// EUP_FUSED_XCD_REMAP=0/1/2 forces the order.
 static int fused_block_order(const Handle *h, int cap)
 {
-    if (const char *e = getenv("EUP_FUSED_XCD_REMAP")) return atoi(e) != 0;
+    if (const char *e = getenv("EUP_FUSED_XCD_REMAP")) {
+        const int v = atoi(e);
+        return v < 0 ? 0 : (v > 2 ? 2 : v);
+    }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->cfg.device) != hipSuccess || cus <= 0)
         cus = 256;
     const double window = (double)cus * (cap > 0 ? cap : 8) * (kBlock / kTile);
     const double plane = (double)h->g.tiles_per_row * h->g.m;
-    return window >= 2.0 * plane ? 0 : 1;
+    return window >= plane ? 2 : 1;
 }
 
 static size_t fused_lds_bytes(const Handle *h) { return occupancy_lds_bytes(h, fused_cap(h)); }

@@ -145,7 +145,7 @@ def interior(p):
     return p[:, 1:-1, 1:-1, 1:-1]
 
 
-@pytest.mark.parametrize("remap", ["0", "1"])
+@pytest.mark.parametrize("remap", ["0", "1", "2"])
 @pytest.mark.parametrize("kind,x_open,shape", [("cylinder", True, (70, 24, 20)),
                                                 ("cylinder", False, (64, 24, 20)),
                                                 ("sponge", False, (100, 24, 20)),
@@ -155,10 +155,14 @@ def interior(p):
                                                 ("random", True, (5, 3, 2)),
                                                 ("random", False, (7, 3, 2)),
                                                 ("random", False, (67, 2, 3)),
-                                                ("random", True, (130, 5, 4))])
+                                                ("random", True, (130, 5, 4)),
+                                                ("cylinder", True, (320, 12, 10)),
+                                                ("random", False, (512, 8, 6))])
 def test_physics_f64_equals_oracle(kind, x_open, shape, remap, monkeypatch):
-    """The fused kernel in both block orders (XCD-contiguous or dispatch order) against the
-    oracle, bit for bit."""
+    """The fused kernel in its three block orders (dispatch order, XCD-contiguous ranges,
+    XCD-owned tile rows) against the oracle, bit for bit.  320 and 512 sites in x give rows of
+    5 and 8 tiles (two blocks per row, so the row order deals out groups of 16 blocks; 320 x 12 x
+    10 also ends in a partial group)."""
     monkeypatch.setenv("EUP_FUSED_XCD_REMAP", remap)
     l, m, n = shape
     announce(f"physics f64 {kind} {shape} x_open={x_open} xcd_remap={remap} vs oracle, 25 steps")
@@ -179,16 +183,17 @@ def test_physics_f64_equals_oracle(kind, x_open, shape, remap, monkeypatch):
     assert not bad.any(), f"{bad.sum()} values differ, max abs {np.abs(a - b).max()}"
 
 
-@pytest.mark.parametrize("shape,x_open", [((128, 48, 48), True), ((200, 24, 24), False)])
+@pytest.mark.parametrize("shape,x_open", [((128, 48, 48), True), ((200, 24, 24), False),
+                                          ((320, 24, 20), True)])
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
 def test_physics_tile_skip_is_exact(storage, shape, x_open, monkeypatch):
     """The pure-fluid tile path against the per-site mixed path (every tile marked mixed), in
-    both block orders."""
+    every block order."""
     l, m, n = shape
     mask, pops = physics_case("sponge", l, m, n, seed=3)
     svisc = (2 * 0.52 - 1) / 6
     out = []
-    for remap in ("0", "1"):
+    for remap in ("0", "1", "2"):
         monkeypatch.setenv("EUP_FUSED_XCD_REMAP", remap)
         for skip in (True, False):
             announce(f"physics {storage} sponge {l}x{m}x{n} x_open={x_open} xcd_remap={remap} "
