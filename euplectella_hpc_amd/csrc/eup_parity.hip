@@ -216,14 +216,26 @@ struct SegMap {
     int tpr;
     uint32_t segments;
     UDiv dtpr, dm;
-    uint32_t chunk; // blocks per XCD (remap)
-    uint32_t remap; // 1: each XCD takes a contiguous range of segments; 0: dispatch order
+    uint32_t chunk; // blocks per XCD (order 1)
+    uint32_t grp;   // blocks of one row, rounded up (order 2)
+    UDiv dgrp;      // n / grp
+    uint32_t order; // 0 dispatch, 1 XCD-contiguous ranges of segments, 2 XCD-owned rows
 };
 
 __global__ __launch_bounds__(256) void k_stream_collide(PArgs a, SegMap s)
 {
+    // block order as in the physics kernel: hardware block b runs on XCD b % 8; order 2 deals
+    // each group of 8 * grp blocks out so that every XCD takes a whole row (the lines that
+    // x-neighbouring segments share through their x-shifted pulls stay in one L2), a last
+    // partial group keeps dispatch order
     const uint32_t b = blockIdx.x;
-    const uint32_t lb = s.remap ? (b & 7u) * s.chunk + (b >> 3) : b;
+    uint32_t lb = b;
+    if (s.order == 1) {
+        lb = (b & 7u) * s.chunk + (b >> 3);
+    } else if (s.order == 2) {
+        const uint32_t r8 = b >> 3, gq = s.dgrp.div(r8), g0 = gq * 8u * s.grp;
+        if (g0 + 8u * s.grp <= gridDim.x) lb = g0 + (b & 7u) * s.grp + (r8 - gq * s.grp);
+    }
     const uint32_t w = __builtin_amdgcn_readfirstlane(lb * 4u + (threadIdx.x >> 6));
     if (w >= s.segments) return;
     const uint32_t rowi = s.dtpr.div(w);
@@ -372,10 +384,16 @@ int parity_step(Handle *h)
     sm.dm = UDiv::make((uint32_t)g.m);
     const uint32_t nb = (sm.segments + 3) / 4;
     sm.chunk = (nb + 7) / 8;
-    // dispatch order: measured 5 % faster than giving each XCD a contiguous range (512 x 256 x 256
-    // sponge); EUP_PARITY_XCD_REMAP=1 selects the remap
-    sm.remap = 0;
-    if (const char *e = getenv("EUP_PARITY_XCD_REMAP")) sm.remap = atoi(e) != 0;
+    sm.grp = (uint32_t)((sm.tpr + 3) / 4);
+    sm.dgrp = UDiv::make(sm.grp);
+    // XCD-owned rows (order 2).  On the 512 x 256 x 256 sponge (MI355X) order 2 ran 17 150-17 180
+    // MLUPS against 17 100 in dispatch order (0), which had measured 5 % faster than contiguous
+    // ranges (1).  EUP_PARITY_XCD_REMAP=0/1/2 forces the order.
+    sm.order = 2;
+    if (const char *e = getenv("EUP_PARITY_XCD_REMAP")) {
+        const int v = atoi(e);
+        sm.order = v < 0 ? 0u : (v > 2 ? 2u : (uint32_t)v);
+    }
     if (h->parity_lds == (size_t)-1) {
         // occupancy cap of the stream-collide (as for the fused FP64 kernel): 3 blocks per CU run
         // about 4 % faster on MI355X than the register-limited occupancy;

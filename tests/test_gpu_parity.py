@@ -39,10 +39,10 @@ def parity_lattice(name, golden):
     return rec, mask, pops, lat
 
 
-@pytest.mark.parametrize("remap", ["0", "1"])
+@pytest.mark.parametrize("remap", ["0", "1", "2"])
 def test_parity_tiny_full_state(golden, remap, monkeypatch):
     """Every site of the state (ghosts included) after 20 steps, bit for bit, with the
-    stream-collide in dispatch order and with each XCD on a contiguous range of segments."""
+    stream-collide in each block order (dispatch order, XCD-contiguous ranges, XCD-owned rows)."""
     monkeypatch.setenv("EUP_PARITY_XCD_REMAP", remap)
     announce(f"parity c0_tiny 16x8x12, 20 steps, xcd_remap={remap}")
     rec, mask, pops, lat = parity_lattice("c0_tiny", golden)
@@ -54,9 +54,9 @@ def test_parity_tiny_full_state(golden, remap, monkeypatch):
     assert not bad.any(), f"{bad.sum()} values differ (first at {np.argwhere(bad)[:3].tolist()})"
 
 
-@pytest.mark.parametrize("name,remap", [("c1_small", "0"), ("c2_ridged", "0"), ("c3_sponge", "0"),
-                                         ("c1_cylinder", "0"), ("c3_sponge", "1"),
-                                         ("c4_scattered", "0"), ("c4_scattered", "1")])
+@pytest.mark.parametrize("name,remap", [("c1_small", "2"), ("c2_ridged", "2"), ("c3_sponge", "2"),
+                                         ("c1_cylinder", "2"), ("c3_sponge", "1"), ("c3_sponge", "0"),
+                                         ("c4_scattered", "2"), ("c4_scattered", "1")])
 def test_parity_state_hashes(golden, name, remap, monkeypatch):
     monkeypatch.setenv("EUP_PARITY_XCD_REMAP", remap)
     rec, mask, pops, lat = parity_lattice(name, golden)
@@ -68,6 +68,28 @@ def test_parity_state_hashes(golden, name, remap, monkeypatch):
             done = t
             got = lat.get_state()
             assert sha(got) == rec["state_sha256"][str(t)], f"{name}: state differs at step {t}"
+
+
+@pytest.mark.parametrize("remap", ["0", "1", "2"])
+@pytest.mark.parametrize("shape", [(320, 12, 10), (512, 8, 6)])
+def test_parity_block_orders_equal_oracle(shape, remap, monkeypatch):
+    """Parity mode in each stream-collide block order against the oracle's serial step, bit for bit
+    (ghosts included), on lattices whose rows span two blocks (320 x 12 x 10 also ends in a
+    partial group of blocks)."""
+    monkeypatch.setenv("EUP_PARITY_XCD_REMAP", remap)
+    l, m, n = shape
+    announce(f"parity cylinder {l}x{m}x{n} xcd_remap={remap} vs oracle, 12 steps")
+    mask, pops = physics_case("cylinder", l, m, n, seed=11)
+    svisc = (2 * 0.6 - 1) / 6
+    ref = pops.copy()
+    orc.parity_steps(ref, mask, orc.omega_from_svisc(svisc), 0.05, 12)
+    with lattice((l, m, n), svisc=svisc, u_inflow=0.05, mode="parity", storage="f64") as lat:
+        lat.set_mask(mask)
+        lat.set_state(pops)
+        lat.step(12)
+        got = lat.get_state()
+    bad = got.view(np.uint64) != ref.view(np.uint64)
+    assert not bad.any(), f"{bad.sum()} values differ (first at {np.argwhere(bad)[:3].tolist()})"
 
 
 @pytest.mark.parametrize("name,nsteps", [("c1_small", 10), ("c2_ridged", 60)])
