@@ -448,7 +448,8 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": {"f64": "f64", "f32": "f32", "f16": "f16-dev/f32", "bf16": "bf16-dev/f32",
                   "f16s": "f16-dev(u_in)/f32", "bf16s": "bf16-dev(u_in)/f32"}[args.storage],
-        "data": "synthetic (seeded sponge geometry, u = inlet + seeded noise)",
+        "data": "synthetic (seeded %s geometry, u = inlet + seeded noise)"
+                % {"cylinder": "cylinder", "ridged": "ridged-cylinder"}.get(wl.get("kind"), "sponge"),
         "config": {"workload": args.workload, "lattice": [wl["l"], wl["m"], wl["n"]],
                    "tau": wl["tau"], "u_in": wl["u_in"], "storage": args.storage,
                    "mode": args.mode, "parallelism": "single GPU",
