@@ -256,7 +256,10 @@ int eup_create(const eup_config *cfg, eup_handle **out)
         delete h;
         return rc;
     }
-    h->device_bytes = (int64_t)(2 * state + sizeof(double) * g.ghosted + g.sites);
+    const size_t outlet = cfg->mode == EUP_MODE_PHYSICS
+                              ? 2 * (cfg->storage == EUP_STORE_F64 ? 8 : 4) * 5 * (size_t)g.m * g.n
+                              : (cfg->mode == EUP_MODE_PARITY ? sizeof(double) * 5 * (size_t)g.m * g.n : 0);
+    h->device_bytes = (int64_t)(2 * state + sizeof(double) * g.ghosted + g.sites + outlet);
     *out = h;
     return 0;
 }

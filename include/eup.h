@@ -86,7 +86,9 @@ typedef struct eup_info {
     int64_t tiles, tiles_fluid, tiles_mixed, tiles_solid; /* 64-site tiles (physics mode) */
     int32_t bbox[6];          /* imin, imax, jmin, jmax, kmin, kmax (build.obs.F90:155-163) */
     int32_t bytes_per_site;   /* algorithmic HBM bytes per updated site per step */
-    int64_t device_bytes;     /* device memory held by the handle */
+    int64_t device_bytes;     /* device memory allocated by eup_create (two state buffers, staging,
+                                 solid flags, outlet planes); per-mask tables and diagnostic
+                                 scratch allocated later are not counted */
 } eup_info;
 
 /* --- lifetime ------------------------------------------------------------------------------ */
