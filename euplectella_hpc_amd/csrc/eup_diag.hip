@@ -50,7 +50,7 @@ __global__ void k_site19(Acc acc, int i, int j, int k, double *out)
 // flux_X.F90:173-242 / flux_Y.F90:168-242 for one site; axis 0 -> sigma_xx, 1 -> sigma_xy
 __device__ double site_sigma(const double *x, double omega, int axis)
 {
-    const double P0 = 1.0 / 3.0, P1 = 1.0 / 18.0, P2 = 1.0 / 36.0, RF = 3.0, QF = 1.5;
+    const double P1 = 1.0 / 18.0, P2 = 1.0 / 36.0, RF = 3.0, QF = 1.5;
     double rho = +x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9] + x[10]
                  + x[11] + x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18];
     double rhoinv = 1.0 / rho;

@@ -622,8 +622,11 @@ int objective_t(Handle *h, int t0, int t1, int K, double norm, double *J, void *
     int c = 0;
     for (int t = 0; t <= t1; ++t) {
         void *cur = work[c];
-        if (t % K == 0)
-            hipMemcpyAsync(ck + S * (size_t)(t / K), cur, S, hipMemcpyDeviceToDevice, h->stream);
+        if (t % K == 0 && hip_check(h, hipMemcpyAsync(ck + S * (size_t)(t / K), cur, S, hipMemcpyDeviceToDevice,
+                                                      h->stream), "checkpoint copy")) {
+            cleanup();
+            return EUP_EHIP;
+        }
         if (t >= t0 && force_into<R>(h, cur, 1.0, acc, acc + 3)) {
             cleanup();
             return EUP_EHIP;
@@ -651,7 +654,10 @@ int objective_t(Handle *h, int t0, int t1, int K, double norm, double *J, void *
             cleanup();
             return fail(h, EUP_EHIP, "eup_grey_objective: out of device memory for the adjoint");
         }
-        hipMemsetAsync(gs, 0, sizeof(double) * N, h->stream);
+        if (hip_check(h, hipMemsetAsync(gs, 0, sizeof(double) * N, h->stream), "memset gradient")) {
+            cleanup();
+            return EUP_EHIP;
+        }
         auto adj = [&](const void *state, const void *lam_in, void *lin, void *lam_out, double w) {
             AdjArgs<R> q;
             q.g = gargs<R>(h, state, nullptr);
@@ -665,7 +671,11 @@ int objective_t(Handle *h, int t0, int t1, int K, double norm, double *J, void *
         // seed at t1: recompute a^{t1} from its checkpoint
         {
             const int cc = t1 / K;
-            hipMemcpyAsync(work[0], ck + S * (size_t)cc, S, hipMemcpyDeviceToDevice, h->stream);
+            if (hip_check(h, hipMemcpyAsync(work[0], ck + S * (size_t)cc, S, hipMemcpyDeviceToDevice, h->stream),
+                          "checkpoint restore")) {
+                cleanup();
+                return EUP_EHIP;
+            }
             int w = 0;
             for (int t = cc * K; t < t1; ++t) {
                 GArgs<R> a = gargs<R>(h, work[w], work[w ^ 1]);
@@ -688,7 +698,11 @@ int objective_t(Handle *h, int t0, int t1, int K, double norm, double *J, void *
         for (int cc = (t1 - 1) / K; cc >= 0; --cc) {
             const int tb = cc * K, te = (cc + 1) * K < t1 ? (cc + 1) * K : t1;
             if (te <= tb) continue; // no step in this segment (t1 >= 1 keeps every segment non-empty)
-            hipMemcpyAsync(seg, ck + S * (size_t)cc, S, hipMemcpyDeviceToDevice, h->stream);
+            if (hip_check(h, hipMemcpyAsync(seg, ck + S * (size_t)cc, S, hipMemcpyDeviceToDevice, h->stream),
+                          "checkpoint restore")) {
+                cleanup();
+                return EUP_EHIP;
+            }
             for (int t = tb + 1; t < te; ++t) {
                 GArgs<R> a = gargs<R>(h, seg + S * (size_t)(t - 1 - tb), seg + S * (size_t)(t - tb));
                 hipLaunchKernelGGL((k_grey_step<R>), grid, dim3(256), 0, h->stream, a);
