@@ -304,6 +304,9 @@ int eup_set_mask(eup_handle *h, const int32_t *obs)
                     if (k > bb[5]) bb[5] = k;
                 }
             }
+    // from here on the device-side mask tables change: a failure leaves the handle without a mask
+    // (eup_step refuses it) rather than with tables that no longer match nsolid
+    h->mask_set = false;
     int rc = 0;
     if (hip_check(h, hipMemcpy(h->solid, sol, g.sites, hipMemcpyHostToDevice), "copy solid"))
         rc = EUP_EHIP;
@@ -364,6 +367,7 @@ int eup_commit_obstacles(eup_handle *h)
         return fail(h, EUP_EINVAL, "eup_commit_obstacles: grey mode takes a solid fraction (eup_set_grey)");
     if (hip_check(h, hipSetDevice(h->cfg.device), "hipSetDevice")) return EUP_EHIP;
     const Geometry &g = h->g;
+    h->mask_set = false; // set again only once the mask tables are rebuilt
     if (!h->obstacles_open && hip_check(h, hipMemsetAsync(h->solid, 0, g.sites, h->stream), "hipMemset solid"))
         return EUP_EHIP;
     int64_t *part = nullptr, *part_h = nullptr;
@@ -629,6 +633,7 @@ int eup_set_grey(eup_handle *h, const double *s)
     if (!s) return fail(h, EUP_EINVAL, "eup_set_grey: null argument");
     for (int64_t x = 0; x < h->g.sites; ++x)
         if (!(s[x] >= 0.0 && s[x] <= 1.0)) return fail(h, EUP_EINVAL, "eup_set_grey: s must lie in [0, 1]");
+    h->grey_set = false; // set again only once the field is rewritten
     rc = grey_set_field(h, s, EUP_STORE_F64, false);
     if (rc) return rc;
     h->grey_set = true;
@@ -642,6 +647,7 @@ int eup_set_grey_device(eup_handle *h, const void *s, int32_t dtype)
     if (rc) return rc;
     if (!s || (dtype != EUP_STORE_F64 && dtype != EUP_STORE_F32))
         return fail(h, EUP_EINVAL, "eup_set_grey_device: null pointer or dtype not F64/F32");
+    h->grey_set = false;
     rc = grey_set_field(h, s, dtype, true);
     if (rc) return rc;
     h->grey_set = true;

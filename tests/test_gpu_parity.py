@@ -487,3 +487,32 @@ def test_init_equilibrium_matches_the_closed_form(mode):
         cu = d3q19.C[q, 0] * ux + d3q19.C[q, 1] * uy
         e = d3q19.W[q] * (1.0 + (3.0 * cu + 1.5 * (3.0 * cu * cu - usq)))
         assert np.abs(got[q][region[1:]] - e[region[1:]]).max() < 1e-17, q
+
+
+@pytest.mark.parametrize("mode", ["parity", "physics"])
+def test_rejected_mask_leaves_the_committed_mask_in_place(mode):
+    """eup_set_mask with a value other than 0 / 1 returns EUP_EINVAL before it touches the device:
+    the mask set before it stays in force (same solid count, bounding box and steps, bit for bit)."""
+    import ctypes
+    from euplectella_hpc_amd import geometry
+    from euplectella_hpc_amd.lib import EUP_OK
+    l, m, n = 48, 16, 8
+    announce(f"{mode} f64 {l}x{m}x{n}: rejected mask keeps the committed one")
+    mask = geometry.cylinder(l, m, n, 3, (12, (m + 1) / 2))
+    pops = geometry.initial_state(l, m, n, 0.04, noise=1e-4, seed=5)
+    bad = np.ascontiguousarray(mask != 0, dtype=np.int32)
+    bad[n // 2, m // 2, l // 2] = 2
+    states = []
+    for reject in (False, True):
+        with lattice((l, m, n), svisc=0.05, u_inflow=0.04, mode=mode, storage="f64") as lat:
+            lat.set_mask(mask)
+            info0 = lat.info()
+            if reject:
+                rc = lat.L.eup_set_mask(lat.h, bad.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+                assert rc != EUP_OK
+                info1 = lat.info()
+                assert info1["nsolid"] == info0["nsolid"] and info1["bbox"] == info0["bbox"]
+            lat.set_state(pops)
+            lat.step(50)
+            states.append(lat.get_state())
+    assert np.array_equal(states[0].view(np.uint64), states[1].view(np.uint64))
