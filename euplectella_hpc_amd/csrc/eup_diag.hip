@@ -422,7 +422,10 @@ static int drag_axis(Handle *h, const int32_t box[6], int axis, double *res)
     if (!clip_faces(h->g, box, axis, &f[0], &f[1], &n1, &n2, &lo, &hi, &k1, &k2)) return 0;
     const int lim_in = axis == 0 ? h->g.m : h->g.l;
     const int lim_f = axis == 0 ? h->g.l : h->g.m;
-    if (lo < 1 || hi > lim_in || k1 < 1 || k2 > h->g.n || f[0] < 1 || f[1] > lim_f)
+    // a box ending at 0 puts its second face on plane 0: the ghost plane of the reference's arrays,
+    // which only parity mode holds; the other layouts have no plane 0
+    const int fmin = h->cfg.mode == EUP_MODE_PARITY ? 0 : 1;
+    if (lo < 1 || hi > lim_in || k1 < 1 || k2 > h->g.n || f[0] < 1 || f[1] < fmin || f[1] > lim_f)
         return fail(h, EUP_ERANGE, "drag box outside the lattice after clipping");
     const int64_t per = (int64_t)(hi - lo + 1) * (k2 - k1 + 1);
     if (per <= 0) return 0;

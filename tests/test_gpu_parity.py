@@ -516,3 +516,34 @@ def test_rejected_mask_leaves_the_committed_mask_in_place(mode):
             lat.step(50)
             states.append(lat.get_state())
     assert np.array_equal(states[0].view(np.uint64), states[1].view(np.uint64))
+
+
+def test_drag_box_ending_at_plane_zero(golden):
+    """A control box whose x range ends at 0 puts its second face on plane 0.  Parity mode holds
+    the reference's ghost plane there and equals the oracle's flux_X / flux_Y bit for bit; the
+    physics and grey layouts have no plane 0 and refuse the box (EUP_ERANGE)."""
+    from euplectella_hpc_amd import geometry
+    from euplectella_hpc_amd.lib import EupError, ERRORS
+    box = [-3, 0, 2, 6, 2, 10]
+    rec, mask, pops, lat = parity_lattice("c0_tiny", golden)
+    announce("drag box ending at plane 0: parity vs oracle, physics and grey refuse")
+    with lat:
+        lat.step(5)
+        state = lat.get_state()
+        fx, fy = lat.drag_cv(box)
+        omega = lat.omega
+    assert fx == orc.flux_x(state, omega, box) and fy == orc.flux_y(state, omega, box)
+    l, m, n = 16, 8, 12
+    p0 = geometry.initial_state(l, m, n, 0.04, noise=1e-4, seed=3)
+    for mode in ("physics", "grey"):
+        with lattice((l, m, n), svisc=0.05, u_inflow=0.04, mode=mode, storage="f64") as lat:
+            if mode == "grey":
+                lat.set_grey(np.zeros((n, m, l)))
+            else:
+                lat.set_mask(np.zeros((n, m, l), np.uint8))
+            lat.set_state(p0)
+            with pytest.raises(EupError) as err:
+                lat.drag_cv(box)
+            assert ERRORS[err.value.code] == "EUP_ERANGE"
+            fx, fy = lat.drag_cv([2, 9, 2, 6, 2, 10])   # a box inside the lattice still works
+            assert np.isfinite(fx) and np.isfinite(fy)
