@@ -161,7 +161,9 @@ int eup_sync(eup_handle *h);
 /* Control-volume stress drag of flux_X/flux_Y (flux_X.F90:50-301, flux_Y.F90:51-282) on the box
  * {xstart, xstop, ystart, ystop, zstart, zstop} (global 1-based indices, drag.F90:48-58).  Per-site
  * stresses are computed on the GPU and summed in the reference's loop order, so the result equals
- * the reference's to the last bit on the same populations. */
+ * the reference's to the last bit on the same populations.  A box whose x (or y) range ends at 0
+ * puts a face on plane 0, the reference's ghost plane: parity mode reads it there, the physics and
+ * grey layouts have no plane 0 and return EUP_ERANGE. */
 int eup_drag_cv(eup_handle *h, const int32_t box[6], double *fluxX, double *fluxY);
 
 /* Momentum-exchange force on the solid, F = sum over fluid sites x and links p with x + c_p solid
