@@ -27,31 +27,37 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        i, d = ctypes.c_int, ctypes.c_double
-        L.orc_parity_steps.argtypes = [_dp, _dp, _ip, i, i, i, d, d, d, i]
-        L.orc_physics_steps.argtypes = [_dp, _dp, _ip, i, i, i, d, d, d, i, i]
-        L.orc_phase_bcond_obs.argtypes = [_dp, _ip, i, i, i]
-        L.orc_phase_bcond_bc.argtypes = [_dp, i, i, i, d]
-        L.orc_phase_movef.argtypes = [_dp, _dp, i, i, i]
-        L.orc_phase_col.argtypes = [_dp, _dp, i, i, i, d, d]
-        L.orc_collide_site.argtypes = [_dp, _dp, d, d]
-        _fp = ctypes.POINTER(ctypes.c_float)
-        L.orc_vtk_velocity.argtypes = [_dp, i, i, i, i, i, _fp]
-        L.orc_vtk_vorticity.argtypes = [_dp, i, i, i, i, _fp]
-        L.orc_flux_x.argtypes = [_dp, i, i, i, d, _ip, _dp]
-        L.orc_flux_y.argtypes = [_dp, i, i, i, d, _ip, _dp]
-        L.orc_probe.argtypes = [_dp, i, i, i, i, i, i, _dp]
-        L.orc_probe.restype = ctypes.c_int
-        L.orc_diagno.argtypes = [_dp, _ip, i, i, i, _dp]
-        L.orc_mem_force.argtypes = [_dp, _ip, i, i, i, i, _dp]
-        L.orc_grey_steps.argtypes = [_dp, _dp, i, i, i, d, d, i, i]
-        L.orc_grey_force.argtypes = [_dp, _dp, i, i, i, d, i, _dp]
-        L.orc_grey_objective.argtypes = [_dp, _dp, i, i, i, d, d, i, i, i, i, d, _dp, _dp]
-        L.orc_collide_adj.argtypes = [_dp, _dp, d, _dp]
-        L.orc_grey_step_adj.argtypes = [_dp, _dp, i, i, i, d, d, i, _dp, _dp, d, _dp]
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def load(path):
+    """A build of eup_oracle.c at ``path`` with its argument types set (the tests also load copies
+    with a planted mistake, to show that a pin catches it)."""
+    L = ctypes.CDLL(path)
+    i, d = ctypes.c_int, ctypes.c_double
+    L.orc_parity_steps.argtypes = [_dp, _dp, _ip, i, i, i, d, d, d, i]
+    L.orc_physics_steps.argtypes = [_dp, _dp, _ip, i, i, i, d, d, d, i, i]
+    L.orc_phase_bcond_obs.argtypes = [_dp, _ip, i, i, i]
+    L.orc_phase_bcond_bc.argtypes = [_dp, i, i, i, d]
+    L.orc_phase_movef.argtypes = [_dp, _dp, i, i, i]
+    L.orc_phase_col.argtypes = [_dp, _dp, i, i, i, d, d]
+    L.orc_collide_site.argtypes = [_dp, _dp, d, d]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.orc_vtk_velocity.argtypes = [_dp, i, i, i, i, i, _fp]
+    L.orc_vtk_vorticity.argtypes = [_dp, i, i, i, i, _fp]
+    L.orc_flux_x.argtypes = [_dp, i, i, i, d, _ip, _dp]
+    L.orc_flux_y.argtypes = [_dp, i, i, i, d, _ip, _dp]
+    L.orc_probe.argtypes = [_dp, i, i, i, i, i, i, _dp]
+    L.orc_probe.restype = ctypes.c_int
+    L.orc_diagno.argtypes = [_dp, _ip, i, i, i, _dp]
+    L.orc_mem_force.argtypes = [_dp, _ip, i, i, i, i, _dp]
+    L.orc_grey_steps.argtypes = [_dp, _dp, i, i, i, d, d, i, i]
+    L.orc_grey_force.argtypes = [_dp, _dp, i, i, i, d, i, _dp]
+    L.orc_grey_objective.argtypes = [_dp, _dp, i, i, i, d, d, i, i, i, i, d, _dp, _dp]
+    L.orc_collide_adj.argtypes = [_dp, _dp, d, _dp]
+    L.orc_grey_step_adj.argtypes = [_dp, _dp, i, i, i, d, d, i, _dp, _dp, d, _dp]
+    return L
 
 
 def _d(a):
@@ -83,12 +89,12 @@ def parity_steps(pops, mask, omega, u_inflow, nsteps, fgrad=0.0):
     return pops
 
 
-def physics_steps(pops, mask, omega, u_inflow, nsteps, fgrad=0.0, x_open=True):
+def physics_steps(pops, mask, omega, u_inflow, nsteps, fgrad=0.0, x_open=True, L=None):
     assert pops.dtype == np.float64 and pops.flags.c_contiguous
     l, m, n = _shape(pops)
     obs = _obs(mask)
     scratch = np.zeros_like(pops)
-    lib().orc_physics_steps(_d(pops), _d(scratch), obs.ctypes.data_as(_ip), l, m, n, omega, fgrad,
+    (L or lib()).orc_physics_steps(_d(pops), _d(scratch), obs.ctypes.data_as(_ip), l, m, n, omega, fgrad,
                             u_inflow, int(bool(x_open)), int(nsteps))
     return pops
 
@@ -200,11 +206,11 @@ def _dense_shape(a):
     return l, m, n
 
 
-def grey_steps(a, s, omega, u_in, nsteps, x_open=True):
+def grey_steps(a, s, omega, u_in, nsteps, x_open=True, L=None):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     l, m, n = _dense_shape(a)
     s = np.ascontiguousarray(s, dtype=np.float64)
-    rc = lib().orc_grey_steps(_d(a), _d(s), l, m, n, omega, u_in, int(bool(x_open)), int(nsteps))
+    rc = (L or lib()).orc_grey_steps(_d(a), _d(s), l, m, n, omega, u_in, int(bool(x_open)), int(nsteps))
     assert rc == 0
     return a
 
