@@ -163,18 +163,132 @@ def test_host_drag_file_matches_reference(golden):
 
 
 @pytest.mark.gpu
-def test_host_physics_f32_run_completes(golden):
+@pytest.mark.parametrize("storage,tol", [("f64", 0.0), ("f32", 2e-6)])
+def test_host_physics_run_matches_the_oracle(golden, storage, tol):
+    """Physics mode through the host: its save dump after 10 steps equals the oracle's physics
+    step on every fluid site (FP64 bit for bit, FP32 within the stated tolerance), and its probe
+    records are the E14.6 text of the same state's probe."""
+    from oracle import oracle as orc
     rec = golden["c1_small"]
     mask, pops = make_inputs(CASES["c1_small"])
-    print("[gpu] Fortran host, physics f32, c1_small 10 steps", flush=True)
+    print(f"[gpu] Fortran host, physics {storage}, c1_small 10 steps vs oracle", flush=True)
+    ref = orc.physics_steps(pops.copy(), mask, orc.omega_from_svisc(rec["svisc"]), rec["u_in"],
+                            rec["steps"])
     with tempfile.TemporaryDirectory() as wd:
-        write_case(wd, rec, mask, pops, icheck=1, itsave=10)
-        res = run_host(wd, {"EUP_MODE": "physics", "EUP_STORAGE": "f32"})
+        write_case(wd, rec, mask, pops, icheck=rec["steps"], itsave=rec["steps"])
+        res = run_host(wd, {"EUP_MODE": "physics", "EUP_STORAGE": storage})
         assert res.returncode == 0, res.stdout + res.stderr
+        _, state = fortio.read_raw(os.path.join(wd, fortio.save_file_name(rec["steps"])),
+                                   (rec["n"], rec["m"], rec["l"]))
         with open(os.path.join(wd, "probe_g.0000.dat")) as fh:
-            rows = [line.split() for line in fh.read().splitlines()]
-        assert len(rows) == rec["steps"]
-        ref = [r[1:5] for r in rec["probe"]]
-        got = np.array([[float(v) for v in r[1:5]] for r in rows])
-        # physics semantics (fresh seams, no frozen edges) differ from parity: loose bound
-        assert np.all(np.isfinite(got)) and np.abs(got - np.array(ref)).max() < 5e-3
+            probe = fh.read().split()
+    fluid = mask == 0
+    got, want = state[:, 1:-1, 1:-1, 1:-1][:, fluid], ref[:, 1:-1, 1:-1, 1:-1][:, fluid]
+    if tol == 0.0:
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    else:
+        assert np.abs(got - want).max() < tol
+    i, j, k = rec["probe_site"]
+    u = orc.probe(ref, i, j, k)
+    assert int(probe[0]) == rec["steps"]
+    vals = [float(v) for v in probe[1:5]]
+    if tol == 0.0:
+        from golden_cases import e14_6
+        assert vals == [e14_6(x) for x in u]       # 6 significant digits of the same doubles
+    else:
+        assert np.abs(np.array(vals) - u).max() < 1e-6
+
+
+# ---------------------------------------------------------------------------------------------
+# the reference's own run decks (CODE/RUN/bgk.*.input), shrunk (tests/golden/make_golden_decks.py)
+# ---------------------------------------------------------------------------------------------
+
+def _decks():
+    import json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden_decks.json")
+    with open(path) as fh:
+        return json.load(fh)
+
+
+DECKS = _decks()
+
+
+def write_deck_case(wd, rec):
+    with open(os.path.join(wd, "bgk.input"), "w") as fh:
+        fh.write(rec["deck_text"])
+    if "obs" in rec:
+        from euplectella_hpc_amd import geometry
+        kind, kw = rec["obs"]
+        kw = {k: tuple(v) if isinstance(v, list) else v for k, v in kw.items()}
+        mask = getattr(geometry, kind)(rec["l"], rec["m"], rec["n"], **kw)
+        assert hashlib.sha256(mask.astype(np.int32).tobytes()).hexdigest() == rec["obs_sha256"]
+        fortio.write_obs(os.path.join(wd, fortio.obs_file_name()), mask)
+
+
+def test_host_rejects_the_list_lines_as_the_reference_does():
+    """The reference's decks assign lists to scalars (svisc = 0.20, 0.04, ...); the namelist read
+    of the reference stops on them, and so does the host's (same deck, same error), before any
+    GPU work."""
+    rec = DECKS["deck_A"]
+    text = rec["deck_text"].replace("svisc=0.20", "svisc=0.20,0.04,0.02,0.01333,0.01")
+    with tempfile.TemporaryDirectory() as wd:
+        with open(os.path.join(wd, "bgk.input"), "w") as fh:
+            fh.write(text)
+        res = run_host(wd, {"HIP_VISIBLE_DEVICES": ""})
+    assert res.returncode != 0
+    assert "NAMELIST input group 'parameters' was not terminated" in res.stdout + res.stderr
+
+
+def test_host_reads_every_shrunk_deck_up_to_the_gpu():
+    """Each shrunk deck parses (proc_x/y/z of several tasks reset to one, as the serial build
+    does) and the host gets as far as creating the GPU state: without a device it stops there
+    with the library's message."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    for name, rec in DECKS.items():
+        with tempfile.TemporaryDirectory() as wd:
+            write_deck_case(wd, rec)
+            res = run_host(wd, {"HIP_VISIBLE_DEVICES": ""})
+        assert res.returncode != 0, name
+        assert "eup_create failed" in res.stdout and "no HIP device" in res.stdout, (name, res.stdout)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(DECKS))
+def test_host_runs_the_reference_deck_and_writes_its_files(name):
+    """The host on a shrunk reference deck (fresh start from rest, obstacle created in the code
+    or read from the list, the -DDRAG drag boxes where the reference build has them): save dump
+    bit for bit, drag and probe records character for character, diagno.dat to rounding, the
+    obstacle dumps byte for byte, bgk.perf with the reference's lines in its order and the
+    profile rows of fort.100 at the same steps."""
+    rec = DECKS[name]
+    print(f"[gpu] Fortran host, parity mode, {name} {rec['l']}x{rec['m']}x{rec['n']}", flush=True)
+    env = {"EUP_MODE": "parity", "EUP_VTK": "none"}
+    if rec["variant"] == "serial":
+        env["EUP_DRAG"] = "1"
+    with tempfile.TemporaryDirectory() as wd:
+        write_deck_case(wd, rec)
+        res = run_host(wd, env)
+        assert res.returncode == 0, res.stdout + res.stderr
+        _, state = fortio.read_raw(os.path.join(wd, fortio.save_file_name(6)), (rec["n"], rec["m"], rec["l"]))
+        assert hashlib.sha256(np.ascontiguousarray(state).tobytes()).hexdigest() == rec["state_sha256"]
+        del state
+        for key, fn in (("drag_text", "drag.0000.dat"), ("probe_text", "probe_g.0000.dat")):
+            with open(os.path.join(wd, fn)) as fh:
+                assert fh.read() == rec[key], fn
+        with open(os.path.join(wd, "diagno.dat")) as fh:
+            got = np.array([float(v) for v in fh.read().split()])
+        want = np.array([float(v) for v in rec["diagno_text"].split()])
+        assert got.shape == want.shape and got[0] == want[0]
+        assert np.all(np.abs(got[1:] - want[1:]) <= 1e-13)
+        for fn, d in rec.get("dumps", {}).items():
+            with open(os.path.join(wd, fn), "rb") as fh:
+                assert hashlib.sha256(fh.read()).hexdigest() == d["sha256"], fn
+        import sys
+        sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+        from make_golden_decks import perf_labels
+        with open(os.path.join(wd, "bgk.perf")) as fh:
+            assert perf_labels(fh.read()) == rec["perf_labels"]
+        with open(os.path.join(wd, "fort.100")) as fh:
+            assert [int(r.split()[0]) for r in fh.read().splitlines() if r.strip()] == rec["profile_steps"]
