@@ -61,6 +61,12 @@ static int validate(const eup_config *c, std::string &why)
         why = "grey mode requires EUP_STORE_F64 or EUP_STORE_F32";
         return EUP_EINVAL;
     }
+    if (c->mode == EUP_MODE_GREY && c->fgrad != 0.0) {
+        // the grey step and its adjoint have no body-force term (CYLINDER_INF runs with fgrad = 0,
+        // hencol.F90:37 after build.bcond.F90:319-339 zeroes u0)
+        why = "grey mode requires fgrad = 0 (no body force in the grey step and its adjoint)";
+        return EUP_EINVAL;
+    }
     if (storage_bytes(c->storage) == 0) {
         why = "unknown storage";
         return EUP_EINVAL;

@@ -646,7 +646,8 @@ int objective_t(Handle *h, int t0, int t1, int K, double norm, double *J, void *
     const double wobj = 1.0 / ((double)(t1 - t0 + 1) * norm);
     *J = sum[0] * wobj;
     if (want_grad) {
-        char *seg = static_cast<char *>(alloc(S * (size_t)K));
+        // a segment holds at most min(K, t1) forward states (t - tb < te - tb <= min(K, t1))
+        char *seg = static_cast<char *>(alloc(S * (size_t)(K < t1 ? K : t1)));
         void *lam = alloc(S);                    // lam^t between the halves of a step
         void *linb[2] = {alloc(S), alloc(S)};    // lin^t, lin^{t-1}
         double *gs = static_cast<double *>(alloc(sizeof(double) * N));

@@ -289,7 +289,11 @@ class Lattice:
 
     # -- diagnostics ----------------------------------------------------------------------------
     def drag_cv(self, box):
+        """Control-volume drag (fluxX, fluxY) on ``box``."""
         b = np.ascontiguousarray(box, dtype=np.int32)
+        if b.shape != (6,):
+            raise ValueError(f"box must hold 6 indices (xstart, xstop, ystart, ystop, zstart, zstop), "
+                             f"got shape {b.shape}")
         fx, fy = ctypes.c_double(), ctypes.c_double()
         self._check(self.L.eup_drag_cv(self.h, b.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                        ctypes.byref(fx), ctypes.byref(fy)))

@@ -76,7 +76,8 @@ typedef struct eup_config {
     int32_t device;           /* HIP device ordinal */
     uint32_t flags;           /* EUP_FLAG_* */
     double omega;             /* 2/(6*svisc + 1), computed by the caller as hencol.F90:30 */
-    double fgrad;             /* x body-force coefficient (hencol.F90:37; 0 for CYLINDER_INF) */
+    double fgrad;             /* x body-force coefficient (hencol.F90:37; 0 for CYLINDER_INF); grey
+                                 mode has no body force and rejects fgrad != 0 */
     double u_inflow;          /* inlet velocity (build.bcond.F90:324) */
 } eup_config;
 

@@ -967,10 +967,13 @@ int orc_grey_objective(const double *a0, const double *s, int l, int m, int n, d
     *J = acc * wobj;
     if (gs) {
         memset(gs, 0, sizeof(double) * G.N);
-        double *seg = (double *)malloc(S * (size_t)K);
+        double *seg = (double *)malloc(S * (size_t)(K < t1 ? K : t1)); /* a segment holds <= min(K, t1) states */
         double *lam = (double *)calloc(19 * G.N, sizeof(double));
         double *lam2 = (double *)malloc(S), *lin = (double *)malloc(S);
-        if (!seg || !lam || !lam2 || !lin) return -1;
+        if (!seg || !lam || !lam2 || !lin) {
+            free(seg); free(lam); free(lam2); free(lin); free(ck); free(cur); free(nxt);
+            return -1;
+        }
         /* seed at t1 */
         {
             int c = t1 / K;
