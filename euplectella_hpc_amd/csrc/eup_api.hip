@@ -562,6 +562,14 @@ int eup_drag_cv(eup_handle *h, const int32_t box[6], double *fluxX, double *flux
     return diag_drag_cv(h, box, fluxX, fluxY);
 }
 
+int eup_drag_cv_noalloc(eup_handle *h, const int32_t box[6], double *fluxX, double *fluxY)
+{
+    int rc = ready(h, "eup_drag_cv_noalloc");
+    if (rc) return rc;
+    if (!box || !fluxX || !fluxY) return fail(h, EUP_EINVAL, "eup_drag_cv_noalloc: null argument");
+    return diag_drag_cv_noalloc(h, box, fluxX, fluxY);
+}
+
 int eup_drag_mem(eup_handle *h, double F[3])
 {
     int rc = ready(h, "eup_drag_mem");

@@ -7,10 +7,18 @@
 // them in the reference's loop order, so the host sums them exactly as flux_X/flux_Y do.  The
 // momentum-exchange force and the global means are wavefront-reduced on the GPU in a fixed order
 // (deterministic run to run).
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
 #include "eup_internal.h"
+
+// Device scratch of eup_drag_cv_noalloc: the per-site stresses of the two faces of one axis,
+// at most kNoAllocFace sites per face (16 MiB per GPU, held for the life of the process).  Host
+// code never passes this name to a kernel or a copy: it is a host-side stand-in; the device
+// address comes from hipGetSymbolAddress, once per device (noalloc_scratch).
+static constexpr int64_t kNoAllocFace = int64_t(1) << 20;
+static __device__ __align__(256) double scratch[2 * kNoAllocFace];
 
 namespace eup {
 
@@ -414,7 +422,10 @@ static bool clip_faces(const Geometry &g, const int32_t box[6], int axis, int *f
     return true;
 }
 
-static int drag_axis(Handle *h, const int32_t box[6], int axis, double *res)
+// fixed != nullptr: the per-site stresses go to that device array (room for 2 * fixed_cap values),
+// and are summed on the host through a stack buffer, so nothing is allocated
+static int drag_axis(Handle *h, const int32_t box[6], int axis, double *res, double *fixed = nullptr,
+                     int64_t fixed_cap = 0)
 {
     *res = 0.0;
     int f[2], lo, hi, k1, k2;
@@ -429,16 +440,41 @@ static int drag_axis(Handle *h, const int32_t box[6], int axis, double *res)
         return fail(h, EUP_ERANGE, "drag box outside the lattice after clipping");
     const int64_t per = (int64_t)(hi - lo + 1) * (k2 - k1 + 1);
     if (per <= 0) return 0;
-    int rc = ensure_diag(h, 2 * per);
-    if (rc) return rc;
+    int rc = 0;
+    double *buf = fixed;
+    if (fixed) {
+        if (per > fixed_cap)
+            return fail(h, EUP_EINVAL, "eup_drag_cv_noalloc: a face of " + std::to_string(per) +
+                                           " sites exceeds the no-allocation scratch (at most " +
+                                           std::to_string(fixed_cap) + " sites per face)");
+    } else {
+        rc = ensure_diag(h, 2 * per);
+        if (rc) return rc;
+        buf = h->diag;
+    }
     for (int ii = 0; ii < 2; ++ii) {
         rc = with_acc(h, [&](auto acc) {
             hipLaunchKernelGGL((k_sigma<decltype(acc)>), dim3((unsigned)((per + 255) / 256)), dim3(256), 0,
-                               h->stream, acc, axis, f[ii], lo, hi, k1, k2, h->cfg.omega,
-                               h->diag + ii * per);
+                               h->stream, acc, axis, f[ii], lo, hi, k1, k2, h->cfg.omega, buf + ii * per);
             return hip_check(h, hipGetLastError(), "sigma") ? EUP_EHIP : 0;
         });
         if (rc) return rc;
+    }
+    if (fixed) {
+        // the same left-to-right sums as below, fed through a fixed host buffer
+        double part[4096];
+        double fl[2] = {0.0, 0.0};
+        for (int ii = 0; ii < 2; ++ii)
+            for (int64_t off = 0; off < per; off += 4096) {
+                const int64_t cnt = per - off < 4096 ? per - off : 4096;
+                if (hip_check(h, hipMemcpyAsync(part, buf + ii * per + off, sizeof(double) * cnt,
+                                                hipMemcpyDeviceToHost, h->stream), "copy sigma") ||
+                    hip_check(h, hipStreamSynchronize(h->stream), "sync"))
+                    return EUP_EHIP;
+                for (int64_t t = 0; t < cnt; ++t) fl[ii] = fl[ii] + part[t];
+            }
+        *res = axis == 0 ? -fl[0] * n1 + fl[1] * n2 : +fl[0] * n1 + fl[1] * n2;
+        return 0;
     }
     std::vector<double> sg(2 * per);
     if (hip_check(h, hipMemcpyAsync(sg.data(), h->diag, sizeof(double) * 2 * per, hipMemcpyDeviceToHost,
@@ -457,6 +493,35 @@ int diag_drag_cv(Handle *h, const int32_t box[6], double *fx, double *fy)
     int rc = drag_axis(h, box, 0, fx);
     if (rc) return rc;
     return drag_axis(h, box, 1, fy);
+}
+
+// Device address of the no-allocation scratch on the handle's device (current when called):
+// looked up once per device and kept.
+static int noalloc_scratch(Handle *h, double **out)
+{
+    static std::mutex mu;
+    static double *ptr[64] = {};
+    const int dev = h->cfg.device;
+    if (dev < 0 || dev >= 64) return fail(h, EUP_EINVAL, "eup_drag_cv_noalloc: device ordinal above 63");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        if (hip_check(h, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))
+            return EUP_EHIP;
+        ptr[dev] = static_cast<double *>(p);
+    }
+    *out = ptr[dev];
+    return 0;
+}
+
+int diag_drag_cv_noalloc(Handle *h, const int32_t box[6], double *fx, double *fy)
+{
+    double *s = nullptr;
+    int rc = noalloc_scratch(h, &s);
+    if (rc) return rc;
+    rc = drag_axis(h, box, 0, fx, s, kNoAllocFace);
+    if (rc) return rc;
+    return drag_axis(h, box, 1, fy, s, kNoAllocFace);
 }
 
 int diag_drag_mem(Handle *h, double F[3])

@@ -102,6 +102,7 @@ int grey_objective(Handle *h, int t0, int t1, int K, double norm, double *J, voi
 // diagnostics (eup_diag.hip)
 int diag_site(Handle *h, int i, int j, int k, double out[19]);
 int diag_drag_cv(Handle *h, const int32_t box[6], double *fx, double *fy);
+int diag_drag_cv_noalloc(Handle *h, const int32_t box[6], double *fx, double *fy);
 int diag_drag_mem(Handle *h, double F[3]);
 int diag_diagno(Handle *h, double out[5]);
 int diag_vtk_velocity(Handle *h, int plane, int index, float *out);

@@ -104,6 +104,13 @@ module eup
          real(c_double), intent(out) :: fx, fy
       end function eup_drag_cv
 
+      integer(c_int) function eup_drag_cv_noalloc(h, box, fx, fy) bind(c, name='eup_drag_cv_noalloc')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), intent(in) :: box(6)
+         real(c_double), intent(out) :: fx, fy
+      end function eup_drag_cv_noalloc
+
       integer(c_int) function eup_drag_mem(h, f) bind(c, name='eup_drag_mem')
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: h

@@ -35,7 +35,8 @@ EXPORTS = ("eup_create", "eup_destroy", "eup_last_error", "eup_set_mask", "eup_s
            "eup_vtk_vorticity", "eup_get_info",
            "eup_stream", "eup_abi_version", "eup_set_grey", "eup_set_grey_device",
            "eup_grey_force", "eup_grey_objective", "eup_grey_objective_device", "eup_membw",
-           "eup_init_equilibrium", "eup_set_omega", "eup_add_obstacles", "eup_commit_obstacles")
+           "eup_init_equilibrium", "eup_set_omega", "eup_add_obstacles", "eup_commit_obstacles",
+           "eup_drag_cv_noalloc")
 
 
 class EupConfig(ctypes.Structure):
@@ -100,6 +101,7 @@ def load():
     L.eup_step_timed.argtypes = [vp, i32, dp]
     L.eup_sync.argtypes = [vp]
     L.eup_drag_cv.argtypes = [vp, ip, dp, dp]
+    L.eup_drag_cv_noalloc.argtypes = [vp, ip, dp, dp]
     L.eup_drag_mem.argtypes = [vp, dp]
     L.eup_probe.argtypes = [vp, i32, i32, i32, dp]
     L.eup_diagno.argtypes = [vp, dp]
@@ -288,15 +290,16 @@ class Lattice:
         self._check(self.L.eup_sync(self.h))
 
     # -- diagnostics ----------------------------------------------------------------------------
-    def drag_cv(self, box):
-        """Control-volume drag (fluxX, fluxY) on ``box``."""
+    def drag_cv(self, box, noalloc=False):
+        """Control-volume drag (fluxX, fluxY) on ``box``; ``noalloc`` calls eup_drag_cv_noalloc."""
         b = np.ascontiguousarray(box, dtype=np.int32)
         if b.shape != (6,):
             raise ValueError(f"box must hold 6 indices (xstart, xstop, ystart, ystop, zstart, zstop), "
                              f"got shape {b.shape}")
         fx, fy = ctypes.c_double(), ctypes.c_double()
-        self._check(self.L.eup_drag_cv(self.h, b.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                       ctypes.byref(fx), ctypes.byref(fy)))
+        fn = self.L.eup_drag_cv_noalloc if noalloc else self.L.eup_drag_cv
+        self._check(fn(self.h, b.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                       ctypes.byref(fx), ctypes.byref(fy)))
         return fx.value, fy.value
 
     def drag_mem(self):

@@ -167,6 +167,16 @@ int eup_sync(eup_handle *h);
  * grey layouts have no plane 0 and return EUP_ERANGE. */
 int eup_drag_cv(eup_handle *h, const int32_t box[6], double *fluxX, double *fluxY);
 
+/* eup_drag_cv without any device allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same box, same results bit for bit; the per-site stresses go to a
+ * fixed device array of the library (2 x 1 048 576 doubles = 16 MiB per GPU, kept until the
+ * process ends) instead of the handle's scratch, and are summed on the host through a fixed
+ * buffer.  A box whose clipped x (or y) face holds more than 1 048 576 sites is rejected with
+ * EUP_EINVAL before anything runs.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (several host threads, several handles on the same device) race. */
+int eup_drag_cv_noalloc(eup_handle *h, const int32_t box[6], double *fluxX, double *fluxY);
+
 /* Momentum-exchange force on the solid, F = sum over fluid sites x and links p with x + c_p solid
  * of 2 c_p f_p(x), from the current (post-collision) populations; wavefront-reduced in FP64.
  * No reference counterpart (the reference only has eup_drag_cv's control-volume drag). */
