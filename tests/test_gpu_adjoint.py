@@ -208,14 +208,14 @@ def test_c4_fp32_adjoint_matches_fp64_adjoint_and_finite_differences():
     rel = np.abs(g32[big] - g64[big]) / np.abs(g64[big])
     assert np.median(rel) < 1e-3 and np.abs(g32 - g64).max() < 1e-2 * np.abs(g64).max()
     # the FP32 gradient on the 64 cells of the FD check, against the FP64 adjoint and against the
-    # central differences of the FP64 objective (bound 5e-3: FP32 rounding of a 2000-step forward
-    # and adjoint sweep; measured worst 1.4e-3 / 1.4e-3)
+    # central differences of the FP64 objective (bound 5e-4: FP32 rounding of a 2000-step forward
+    # and adjoint sweep; measured on MI355X worst 6.6e-5 / 6.6e-5, median 1.9e-5)
     g32c, g64c, fds = g32.ravel()[cells], g64.ravel()[cells], np.array(fds)
     rel64 = np.abs(g32c - g64c) / np.abs(g64c)
     relfd = np.abs(g32c - fds) / np.abs(fds)
     announce(f"C4 FP32 gradient on the 64 cells: worst relative error {rel64.max():.2e} vs the FP64 "
              f"adjoint, {relfd.max():.2e} vs central differences (median {np.median(rel64):.2e})")
-    assert rel64.max() < 5e-3 and relfd.max() < 5e-3
+    assert rel64.max() < 5e-4 and relfd.max() < 5e-4
 
 
 def test_torch_autograd_function_returns_the_adjoint_gradient():

@@ -63,16 +63,18 @@ def compare_with_f64(name, storages, steps, pop_tol, cd_tol):
 
 @pytest.mark.timeout(600)
 def test_c2_ridged_256x128x128_reduced_storage_against_fp64():
+    # measured on MI355X: max |df| 1.4e-6 / 8.4e-5 / 9.8e-4, |dC_D|/C_D 2.3e-5 / 2.5e-4 / 5.7e-3
     compare_with_f64("ridged_256x128x128", ["f32", "f16", "bf16"], 100,
-                     pop_tol={"f32": 1e-5, "f16": 2e-3, "bf16": 2e-2},
-                     cd_tol={"f32": 1e-4, "f16": 2e-2, "bf16": 1e-1})
+                     pop_tol={"f32": 1e-5, "f16": 5e-4, "bf16": 5e-3},
+                     cd_tol={"f32": 2e-4, "f16": 2e-3, "bf16": 3e-2})
 
 
 @pytest.mark.timeout(600)
 def test_c3_sponge_512x256x256_reduced_storage_against_fp64():
+    # measured on MI355X: max |df| 2.5e-6 / 2.4e-4, |dC_D|/C_D 3.5e-5 / 6.2e-4
     compare_with_f64("sponge_512x256x256", ["f32", "f16"], 100,
-                     pop_tol={"f32": 5e-5, "f16": 5e-3},
-                     cd_tol={"f32": 5e-4, "f16": 5e-2})
+                     pop_tol={"f32": 2e-5, "f16": 1e-3},
+                     cd_tol={"f32": 2e-4, "f16": 3e-3})
 
 
 @pytest.mark.timeout(900)
@@ -115,4 +117,5 @@ def test_c5_sponge_1536x768x768_fp32_and_fp16_at_full_size():
     du = np.abs(probes["f16"][:, :3] - probes["f32"][:, :3]).max()
     drho = np.abs(probes["f16"][:, 3] - probes["f32"][:, 3]).max()
     announce(f"C5 FP16 vs FP32 at {len(sites)} sites: max |du| = {du:.3e}, max |drho| = {drho:.3e}")
-    assert du < 1e-3 and drho < 1e-3
+    # measured on MI355X: 4.5e-5 / 8.4e-5
+    assert du < 5e-4 and drho < 5e-4
