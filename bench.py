@@ -2,8 +2,11 @@
 
 Workload (default): the reticular-sponge lattice 512 x 256 x 256 (33.5 M sites), tau = 0.505,
 u_in = 0.06, inlet/outlet in x, periodic y/z, physics-mode fused collide+stream kernel with FP32
-population storage; the same run with FP16 deviation storage is reported beside it.  Inputs are
-synthetic (seeded generators), resident in HBM before the timed region.
+population storage; the same run with FP16 deviation storage is reported beside it (``secondary``),
+and with the e3m12 16-bit deviation storage (``also``).  Inputs are synthetic (seeded generators),
+resident in HBM before the timed region.  The metric's qualifier, |dC_D|/C_D <= 1e-3 against FP64,
+is measured for each storage in the same run (``accuracy``; ``qualified_16bit`` names the 16-bit
+storage that meets it at the workload's own tau, with its speed-up over FP32).
 
 A step is one fused collide+stream launch over the whole lattice.  MLUPS follows the reference's
 definition (all l*m*n sites per second, CODE/SRC/finalize.F90:107-108).  The roofline figure uses
@@ -87,12 +90,13 @@ def workload_mask(wl, nz=None):
 
 
 def storage_bytes(storage):
-    return {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f16s": 2, "bf16s": 2}[storage]
+    return {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "e3m12": 2, "f16s": 2, "bf16s": 2,
+            "e3m12s": 2}[storage]
 
 
 def parse_storage(spec):
-    """'f16s' / 'bf16s' = FP16 / BF16 deviation from the free-stream equilibrium."""
-    return (spec[:-1], True) if spec in ("f16s", "bf16s") else (spec, False)
+    """'f16s' / 'bf16s' / 'e3m12s' = the deviation from the free-stream equilibrium."""
+    return (spec[:-1], True) if spec in ("f16s", "bf16s", "e3m12s") else (spec, False)
 
 
 def build_lattice(wl, storage, mask, chain=True, mode="physics"):
@@ -490,10 +494,13 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="sponge_512x256x256", choices=sorted(WORKLOADS))
-    ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16", "bf16", "f16s", "bf16s"])
+    ap.add_argument("--storage", default="f32",
+                    choices=["f64", "f32", "f16", "bf16", "e3m12", "f16s", "bf16s", "e3m12s"])
     ap.add_argument("--mode", default="physics", choices=["physics", "parity"],
                     help="parity: the reference's serial FP64 step bit for bit (storage f64, no secondary)")
     ap.add_argument("--secondary", default="f16", help="second storage to report ('' = none)")
+    ap.add_argument("--also", default="e3m12",
+                    help="further storages timed beside the secondary, comma-separated ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain MIB-MiB device copy (bandwidth / counter calibration)")
@@ -502,6 +509,8 @@ def main(argv=None):
     ap.add_argument("--accuracy", type=int, default=0, metavar="STEPS",
                     help="only run the C_D / probe accuracy study of f32, f16, bf16 vs f64")
     ap.add_argument("--every", type=int, default=100, help="drag sampling interval (--accuracy)")
+    ap.add_argument("--accuracy-storages", default="f32,f16,f16s,bf16,bf16s,e3m12,e3m12s",
+                    help="storages compared with FP64 by --accuracy")
     ap.add_argument("--no-chain", action="store_true",
                     help="skip the higher-viscosity start-up chain (short profiling runs)")
     ap.add_argument("--no-streams", action="store_true",
@@ -525,7 +534,7 @@ def main(argv=None):
                                    not args.no_chain)))
         return 0
     if args.accuracy:
-        r = accuracy(WORKLOADS[args.workload], ["f32", "f16", "f16s", "bf16", "bf16s"], args.accuracy,
+        r = accuracy(WORKLOADS[args.workload], args.accuracy_storages.split(","), args.accuracy,
                      args.every)
         print(json.dumps(r))
         return 0
@@ -559,6 +568,12 @@ def main(argv=None):
         sec = run_one(wl, args.secondary, mask, args.steps, args.warmup, not args.no_chain)
         print(f"[bench] {args.secondary}: {sec['mlups']:.0f} MLUPS, kernel {sec['kernel_ms']:.3f} ms",
               file=sys.stderr, flush=True)
+    also = {}
+    for st in ([x for x in args.also.split(",") if x and x not in (args.storage, args.secondary)]
+               if args.mode == "physics" else []):
+        also[st] = run_one(wl, st, mask, args.steps, args.warmup, not args.no_chain)
+        print(f"[bench] {st}: {also[st]['mlups']:.0f} MLUPS, kernel {also[st]['kernel_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
     streams = None if args.no_streams else hbm_streams()
     traffic, traffic_src = (profiled_traffic(args.workload, args.storage) if args.mode == "physics"
                             else (None, None))
@@ -575,7 +590,8 @@ def main(argv=None):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"f64": "f64", "f32": "f32", "f16": "f16-dev/f32", "bf16": "bf16-dev/f32",
-                  "f16s": "f16-dev(u_in)/f32", "bf16s": "bf16-dev(u_in)/f32"}[args.storage],
+                  "e3m12": "e3m12-dev/f32", "f16s": "f16-dev(u_in)/f32", "bf16s": "bf16-dev(u_in)/f32",
+                  "e3m12s": "e3m12-dev(u_in)/f32"}[args.storage],
         "data": "synthetic (seeded %s geometry, u = inlet + seeded noise)"
                 % {"cylinder": "cylinder", "ridged": "ridged-cylinder"}.get(wl.get("kind"), "sponge"),
         "config": {"workload": args.workload, "lattice": [wl["l"], wl["m"], wl["n"]],
@@ -605,15 +621,28 @@ def main(argv=None):
                             "frac": round(sec["frac"], 4),
                             "speedup_vs_primary": round(sec["mlups"] / prim["mlups"], 3),
                             "finite_after_run": sec["finite"]}
+    for st, r in also.items():
+        out.setdefault("also", {})[st] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 4),
+                                          "achieved_gbs": round(r["achieved_gbs"], 1),
+                                          "frac": round(r["frac"], 4),
+                                          "speedup_vs_primary": round(r["mlups"] / prim["mlups"], 3),
+                                          "finite_after_run": r["finite"]}
     if args.mode == "physics" and not args.no_accuracy:
-        acc = accuracy_qualifier(wl, mask, [x for x in (args.storage, args.secondary) if x and x != "f64"])
+        acc = accuracy_qualifier(wl, mask, [x for x in [args.storage, args.secondary] + list(also)
+                                            if x and x != "f64"])
         out["accuracy"] = _rounded(acc)
-        for key, st in (("primary", args.storage), ("secondary", args.secondary)):
-            if st in acc:
-                tgt = out if key == "primary" else out.get("secondary")
-                if tgt is not None:
-                    tgt["accuracy_met_at_workload_tau"] = acc[st]["met_at_workload_tau"]
-                    tgt["accuracy_met_at_tau"] = acc[st]["met_at_tau"]
+        targets = [(args.storage, out), (args.secondary, out.get("secondary"))] + \
+                  [(st, out["also"][st]) for st in also]
+        for st, tgt in targets:
+            if st in acc and tgt is not None:
+                tgt["accuracy_met_at_workload_tau"] = acc[st]["met_at_workload_tau"]
+                tgt["accuracy_met_at_tau"] = acc[st]["met_at_tau"]
+        # the 16-bit storages that meet the metric's qualifier at the workload's own tau
+        sixteen = [(st, tgt) for st, tgt in targets[1:] if tgt is not None and storage_bytes(st) == 2]
+        met = [(st, tgt) for st, tgt in sixteen if tgt.get("accuracy_met_at_workload_tau")]
+        out["qualified_16bit"] = ({"storage": met[0][0], "speedup_vs_primary": met[0][1]["speedup_vs_primary"],
+                                   "rel_dcd": acc["windows"][0][met[0][0]]["rel_dcd"], "tau": wl["tau"]}
+                                  if met else None)
     print(json.dumps(out))
     return 0
 

@@ -36,6 +36,7 @@ static size_t storage_bytes(int storage)
     case EUP_STORE_F32: return 4;
     case EUP_STORE_F16: return 2;
     case EUP_STORE_BF16: return 2;
+    case EUP_STORE_E3M12: return 2;
     }
     return 0;
 }

@@ -44,7 +44,7 @@ template <typename S> struct PhysAcc {
     int l, m, n;
     __device__ __forceinline__ double operator()(int q, int i, int j, int k) const
     {
-        return (double)Store<S>::load(base[ix(q, i - 1, j - 1, k - 1)], ref.r[q]);
+        return (double)Store<S>::load(base[ix(q, i - 1, j - 1, k - 1)], ref.r[q], q);
     }
 };
 
@@ -357,6 +357,7 @@ template <typename F> int with_acc(Handle *h, F &&f)
     case EUP_STORE_F32: return f(phys_acc<float>(h));
     case EUP_STORE_F16: return f(phys_acc<_Float16>(h));
     case EUP_STORE_BF16: return f(phys_acc<bf16_t>(h));
+    case EUP_STORE_E3M12: return f(phys_acc<e3m12_t>(h));
     }
     return fail(h, EUP_EINVAL, "unknown storage");
 }

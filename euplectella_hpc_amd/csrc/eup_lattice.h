@@ -276,34 +276,35 @@ struct bf16_t {
 template <typename S> struct Store;
 
 // `ref` is the reference value of the population for deviation storage (w_q, or the free-stream
-// equilibrium of that population); full-value storages ignore it.
+// equilibrium of that population); full-value storages ignore it.  `q` is the population (0..18):
+// only the weight-scaled format (e3m12_t) uses it, as a compile-time constant at every call site.
 template <> struct Store<double> {
     using compute = double;
     static constexpr bool deviation = false;
-    static __device__ __forceinline__ double load(double v, float) { return v; }
-    static __device__ __forceinline__ double store(double f, float) { return f; }
-    static __device__ __forceinline__ double raw_load(double v) { return v; }
-    static __device__ __forceinline__ double raw_store(double f) { return f; }
+    static __device__ __forceinline__ double load(double v, float, int) { return v; }
+    static __device__ __forceinline__ double store(double f, float, int) { return f; }
+    static __device__ __forceinline__ double raw_load(double v, int) { return v; }
+    static __device__ __forceinline__ double raw_store(double f, int) { return f; }
 };
 
 template <> struct Store<float> {
     using compute = float;
     static constexpr bool deviation = false;
-    static __device__ __forceinline__ float load(float v, float) { return v; }
-    static __device__ __forceinline__ float store(float f, float) { return f; }
-    static __device__ __forceinline__ float raw_load(float v) { return v; }
-    static __device__ __forceinline__ float raw_store(float f) { return f; }
+    static __device__ __forceinline__ float load(float v, float, int) { return v; }
+    static __device__ __forceinline__ float store(float f, float, int) { return f; }
+    static __device__ __forceinline__ float raw_load(float v, int) { return v; }
+    static __device__ __forceinline__ float raw_store(float f, int) { return f; }
 };
 
 // deviation f - ref in binary16; float -> half conversion rounds to nearest even
 template <> struct Store<_Float16> {
     using compute = float;
     static constexpr bool deviation = true;
-    static __device__ __forceinline__ float load(_Float16 v, float ref) { return ref + (float)v; }
-    static __device__ __forceinline__ _Float16 store(float f, float ref) { return (_Float16)(f - ref); }
+    static __device__ __forceinline__ float load(_Float16 v, float ref, int) { return ref + (float)v; }
+    static __device__ __forceinline__ _Float16 store(float f, float ref, int) { return (_Float16)(f - ref); }
     // the stored deviation itself (kernels that compute on deviations)
-    static __device__ __forceinline__ float raw_load(_Float16 v) { return (float)v; }
-    static __device__ __forceinline__ _Float16 raw_store(float d) { return (_Float16)d; }
+    static __device__ __forceinline__ float raw_load(_Float16 v, int) { return (float)v; }
+    static __device__ __forceinline__ _Float16 raw_store(float d, int) { return (_Float16)d; }
 };
 
 // deviation f - ref in bfloat16 with round-to-nearest-even
@@ -319,10 +320,48 @@ template <> struct Store<bf16_t> {
         b.bits = __builtin_bit_cast(uint16_t, h);
         return b;
     }
-    static __device__ __forceinline__ float load(bf16_t v, float ref) { return ref + to_float(v); }
-    static __device__ __forceinline__ bf16_t store(float f, float ref) { return from_float(f - ref); }
-    static __device__ __forceinline__ float raw_load(bf16_t v) { return to_float(v); }
-    static __device__ __forceinline__ bf16_t raw_store(float d) { return from_float(d); }
+    static __device__ __forceinline__ float load(bf16_t v, float ref, int) { return ref + to_float(v); }
+    static __device__ __forceinline__ bf16_t store(float f, float ref, int) { return from_float(f - ref); }
+    static __device__ __forceinline__ float raw_load(bf16_t v, int) { return to_float(v); }
+    static __device__ __forceinline__ bf16_t raw_store(float d, int) { return from_float(d); }
+};
+
+// Deviation f_q - ref_q divided by w_q, as a 16-bit float with 1 sign, 3 exponent and 12 mantissa
+// bits: binades [2^-5, 2^-4) .. [2, 4) (in units of w_q) and below them a fixed step of 2^-17 w_q.
+// Deviations of BGK populations stay well inside |f_q - ref_q| < 2 w_q, where IEEE binary16 spends
+// 5 exponent bits on 30 binades it never uses; this keeps 12 significant bits instead of 11 and
+// an absolute step of w_q 2^-17 near zero.  The top code (all ones) is NaN: a deviation of 4 w_q or
+// more, or a non-finite one, stores NaN rather than a saturated value, so a run that leaves the
+// format's range shows up as non-finite.
+//   encode: v = d 2^-121 / w_q puts the format's subnormal range onto FP32 denormals (kept by the
+//           kernels: float denormal mode preserved), so the FP32 bits of |v| are the code scaled by
+//           2^11: round to nearest even at bit 11 and shift;
+//   decode: the code shifted back into FP32 bits, times w_q 2^121.
+struct e3m12_t {
+    uint16_t bits;
+};
+
+template <> struct Store<e3m12_t> {
+    using compute = float;
+    static constexpr bool deviation = true;
+    static __device__ __forceinline__ e3m12_t raw_store(float d, int q)
+    {
+        // |d| through the multiply's input modifier, the sign from d's own bits
+        const uint32_t a = __float_as_uint(__builtin_fabsf(d) * (0x1p-121f / weight<float>(q)));
+        uint32_t c = (a + 0x3FFu + ((a >> 11) & 1u)) >> 11;
+        c = c < 0x7FFFu ? c : 0x7FFFu;
+        e3m12_t r;
+        r.bits = (uint16_t)(((__float_as_uint(d) >> 16) & 0x8000u) | c);
+        return r;
+    }
+    static __device__ __forceinline__ float raw_load(e3m12_t x, int q)
+    {
+        const uint32_t b = x.bits;
+        const float v = __uint_as_float(((b & 0x8000u) << 16) | ((b & 0x7FFFu) << 11)) * (weight<float>(q) * 0x1p121f);
+        return (b & 0x7FFFu) == 0x7FFFu ? __builtin_nanf("") : v;
+    }
+    static __device__ __forceinline__ float load(e3m12_t v, float ref, int q) { return ref + raw_load(v, q); }
+    static __device__ __forceinline__ e3m12_t store(float f, float ref, int q) { return raw_store(f - ref, q); }
 };
 
 } // namespace eup

@@ -63,8 +63,8 @@ template <typename S> constexpr bool kOnStored = std::is_same_v<typename Store<S
 template <typename S>
 __device__ __forceinline__ typename Store<S>::compute ld(const XArgs<S> &a, const S *p, int q)
 {
-    if constexpr (kOnStored<S>) return Store<S>::raw_load(*p);
-    else return Store<S>::load(*p, a.ref.r[q]);
+    if constexpr (kOnStored<S>) return Store<S>::raw_load(*p, q);
+    else return Store<S>::load(*p, a.ref.r[q], q);
 }
 
 template <int... Qs> struct QSeq {};
@@ -148,8 +148,9 @@ __device__ __forceinline__ void pull(const XArgs<S> &a, const Wave &w, uint32_t 
                 corr = a.bgk.bb[Q];
             }
         }
-        if constexpr (MIXED && Store<S>::deviation) x[Q] = Store<S>::raw_load(*p) + corr;
-        else x[Q] = Store<S>::raw_load(*p);
+        // (the bounce-back path is taken by FP64 only: the other storages use the at-once form)
+        if constexpr (MIXED && Store<S>::deviation) x[Q] = Store<S>::raw_load(*p, Q) + corr;
+        else x[Q] = Store<S>::raw_load(*p, Q);
     } else {
         float ref = a.ref.r[Q];
         if constexpr (MIXED) {
@@ -159,7 +160,7 @@ __device__ __forceinline__ void pull(const XArgs<S> &a, const Wave &w, uint32_t 
                 ref = a.ref.r[op];
             }
         }
-        x[Q] = Store<S>::load(*p, ref);
+        x[Q] = Store<S>::load(*p, ref, Q);
     }
 }
 
@@ -226,7 +227,7 @@ __device__ __forceinline__ void x_edges(const XArgs<S> &a, const Wave &w, uint32
 template <typename S> __device__ __forceinline__ void store_val(S *p, S v)
 {
 #if EUP_NT_STORE
-    if constexpr (std::is_same_v<S, bf16_t>)
+    if constexpr (std::is_same_v<S, bf16_t> || std::is_same_v<S, e3m12_t>)
         __builtin_nontemporal_store(v.bits, reinterpret_cast<uint16_t *>(p));
     else
         __builtin_nontemporal_store(v, p);
@@ -239,8 +240,8 @@ template <typename S, int Q>
 __device__ __forceinline__ void put(const XArgs<S> &a, char *tb, uint32_t o0, const typename Store<S>::compute *x)
 {
     S *p = reinterpret_cast<S *>(tb + (int64_t)Q * kTile * sizeof(S) + (uint64_t)o0);
-    if constexpr (kOnStored<S>) store_val<S>(p, Store<S>::raw_store(x[Q]));
-    else store_val<S>(p, Store<S>::store(x[Q], a.ref.r[Q]));
+    if constexpr (kOnStored<S>) store_val<S>(p, Store<S>::raw_store(x[Q], Q));
+    else store_val<S>(p, Store<S>::store(x[Q], a.ref.r[Q], Q));
 }
 
 template <typename S, int... Qs>
@@ -473,7 +474,7 @@ __global__ void k_from_staging(const double *__restrict__ stg, S *__restrict__ d
     const int j = (int)(r % m), k = (int)(r / m);
     const double v = stg[(int64_t)(i + 1) + (int64_t)(l + 2) * ((int64_t)(j + 1) + (int64_t)(m + 2) * (k + 1))];
     using R = typename Store<S>::compute;
-    const S sv = Store<S>::store((R)v, ref);
+    const S sv = Store<S>::store((R)v, ref, q);
     const int64_t e = P(q, i, j, k);
     d0[e] = sv;
     d1[e] = sv;
@@ -490,7 +491,7 @@ __global__ void k_to_staging(const S *__restrict__ src, double *__restrict__ stg
     const int64_t r = t / l;
     const int j = (int)(r % m), k = (int)(r / m);
     stg[(int64_t)(i + 1) + (int64_t)(l + 2) * ((int64_t)(j + 1) + (int64_t)(m + 2) * (k + 1))] =
-        (double)Store<S>::load(src[P(q, i, j, k)], ref);
+        (double)Store<S>::load(src[P(q, i, j, k)], ref, q);
 }
 
 // counter-based uniform in [-1, 1) (splitmix64 finaliser)
@@ -524,7 +525,7 @@ __global__ void k_init_eq(S *base, PopIndex P, int l, int m, int64_t nsites, dou
         const double cu = cx[q] * ux + cy[q] * uy + cz[q] * uz;
         const double f = weight<double>(q) * (1.0 + (3.0 * cu + 1.5 * (3.0 * cu * cu - usq)));
         using R = typename Store<S>::compute;
-        base[P(q, i, j, k)] = Store<S>::store((R)f, ref.r[q]);
+        base[P(q, i, j, k)] = Store<S>::store((R)f, ref.r[q], q);
     }
 }
 
@@ -546,7 +547,7 @@ __global__ void k_outlet_refresh(XArgs<S> a, typename Store<S>::compute *o0,
     P.tiled = 1;
     R x[19], o[5];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) x[q] = Store<S>::load(a.A[P(q, a.l - 1, j, k)], a.ref.r[q]);
+    for (int q = 0; q < 19; ++q) x[q] = Store<S>::load(a.A[P(q, a.l - 1, j, k)], a.ref.r[q], q);
     outlet_eq<R>(x, o);
 #pragma unroll
     for (int u = 0; u < 5; ++u) {
@@ -743,6 +744,7 @@ template <typename F> int dispatch(Handle *h, F &&f)
     case EUP_STORE_F32: return f((float *)nullptr);
     case EUP_STORE_F16: return f((_Float16 *)nullptr);
     case EUP_STORE_BF16: return f((bf16_t *)nullptr);
+    case EUP_STORE_E3M12: return f((e3m12_t *)nullptr);
     }
     return fail(h, EUP_EINVAL, "unknown storage");
 }

@@ -36,7 +36,7 @@
 ! build.bcond.F90:319-339).  The serial build runs one task whatever proc_x, proc_y, proc_z say
 ! (input.F90:64-72).  The step semantics and storage come from the environment:
 !   EUP_MODE    = parity (default; the reference's serial FP64 step bit for bit) | physics
-!   EUP_STORAGE = f64 (default) | f32 | f16 | bf16   (physics mode)
+!   EUP_STORAGE = f64 (default) | f32 | f16 | bf16 | e3m12   (physics mode)
 !   EUP_DEVICE  = HIP device ordinal (default 0)
 program bgk3d_eup
    use iso_c_binding
@@ -109,6 +109,7 @@ program bgk3d_eup
    case ('f32'); cfg%storage = EUP_STORE_F32
    case ('f16'); cfg%storage = EUP_STORE_F16
    case ('bf16'); cfg%storage = EUP_STORE_BF16
+   case ('e3m12'); cfg%storage = EUP_STORE_E3M12
    end select
    cfg%device = 0
    call get_environment_variable('EUP_DEVICE', env)

@@ -9,7 +9,7 @@ module eup
    implicit none
 
    integer(c_int32_t), parameter :: EUP_STORE_F64 = 0, EUP_STORE_F32 = 1
-   integer(c_int32_t), parameter :: EUP_STORE_F16 = 2, EUP_STORE_BF16 = 3
+   integer(c_int32_t), parameter :: EUP_STORE_F16 = 2, EUP_STORE_BF16 = 3, EUP_STORE_E3M12 = 4
    integer(c_int32_t), parameter :: EUP_MODE_PARITY = 0, EUP_MODE_PHYSICS = 1, EUP_MODE_GREY = 2
    integer(c_int32_t), parameter :: EUP_FACE_PERIODIC = 0, EUP_FACE_INFLOW = 4
    integer(c_int32_t), parameter :: EUP_FACE_OUTFLOW = 5

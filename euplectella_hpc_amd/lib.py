@@ -20,8 +20,9 @@ LIB_PATH = os.environ.get("EUP_LIBRARY") or os.path.join(HERE, "libeup.so")
 EUP_OK = 0
 ERRORS = {-1: "EUP_EINVAL", -2: "EUP_EHIP", -3: "EUP_ESTATE", -4: "EUP_ERANGE"}
 
-STORE_F64, STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2, 3
-STORAGE = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
+STORE_F64, STORE_F32, STORE_F16, STORE_BF16, STORE_E3M12 = 0, 1, 2, 3, 4
+STORAGE = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16,
+           "e3m12": STORE_E3M12}
 MODE_PARITY, MODE_PHYSICS, MODE_GREY = 0, 1, 2
 MODES = {"parity": MODE_PARITY, "physics": MODE_PHYSICS, "grey": MODE_GREY}
 FACE_PERIODIC, FACE_INFLOW, FACE_OUTFLOW = 0, 4, 5
@@ -144,7 +145,7 @@ class Lattice:
     """One D3Q19 lattice on one GPU (a libeup handle).
 
     ``mode`` is "parity" (the reference's serial FP64 step, bit for bit) or "physics" (fused
-    pull kernel, ``storage`` f64 / f32 / f16 / bf16).  ``x_open`` selects inlet/outlet x faces
+    pull kernel, ``storage`` f64 / f32 / f16 / bf16 / e3m12).  ``x_open`` selects inlet/outlet x faces
     (the CYLINDER_INF build) or a periodic x axis.
     """
 

@@ -44,6 +44,9 @@ extern "C" {
 #define EUP_STORE_F32 1   /* full f in FP32, FP32 collision */
 #define EUP_STORE_F16 2   /* deviation f_q - w_q in IEEE binary16, FP32 collision */
 #define EUP_STORE_BF16 3  /* deviation f_q - w_q in bfloat16, FP32 collision */
+#define EUP_STORE_E3M12 4 /* deviation (f_q - w_q) / w_q as a 16-bit float with 3 exponent and 12
+                             mantissa bits (range below 4, step 2^-17 near zero; out of range
+                             stores NaN), FP32 collision */
 
 /* time-step semantics */
 #define EUP_MODE_PARITY 0  /* the reference's serial step, bit for bit (FP64, ghosted arrays) */

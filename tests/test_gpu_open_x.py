@@ -49,11 +49,12 @@ def bits(a):
 # FP64 1.3e-15 (rounding accumulated over 200 steps; the oracle's step, bit for bit); FP32 5.5e-7;
 # FP16 / BF16 deviations from w_q (about 1e-2, rounded to 11 / 8 significant bits) 8.8e-6 /
 # 2.0e-4; deviations from the free stream itself 1.3e-7 / 6.4e-7; grey FP32 3.7e-7.
-FREE_TOL = {"f64": 1e-14, "f32": 2e-6, "f16": 2e-5, "bf16": 5e-4}
+FREE_TOL = {"f64": 1e-14, "f32": 2e-6, "f16": 2e-5, "bf16": 5e-4, "e3m12": 2e-5}
 
 
 @pytest.mark.parametrize("storage,shifted", [("f64", False), ("f32", False), ("f16", False),
-                                             ("f16", True), ("bf16", False), ("bf16", True)])
+                                             ("f16", True), ("bf16", False), ("bf16", True),
+                                             ("e3m12", False), ("e3m12", True)])
 def test_physics_free_stream_is_stationary_with_open_x(storage, shifted):
     l, m, n = 150, 6, 5
     announce(f"physics {storage}{'s' if shifted else ''} free stream {l}x{m}x{n}, open x, 200 steps")
@@ -129,10 +130,10 @@ def test_grey_f64_binary_open_x_equals_the_reference_step_away_from_the_edges(re
 
 # reduced storage against the reference step on the same sites after 8 steps (measured on MI355X:
 # 2.5e-7, 1.7e-5, 1.5e-4)
-REGION_TOL = {"f32": 1e-6, "f16": 5e-5, "bf16": 5e-4}
+REGION_TOL = {"f32": 1e-6, "f16": 5e-5, "bf16": 5e-4, "e3m12": 5e-5}
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16", "e3m12"])
 def test_reduced_storage_open_x_near_the_reference_step(region, storage):
     shape, mask, pops, ref, cmp = region
     announce(f"physics {storage} open x vs reference step, 150x28x26, {STEPS} steps")

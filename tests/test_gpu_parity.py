@@ -207,7 +207,7 @@ def test_physics_f64_equals_oracle(kind, x_open, shape, remap, monkeypatch):
 
 @pytest.mark.parametrize("shape,x_open", [((128, 48, 48), True), ((200, 24, 24), False),
                                           ((320, 24, 20), True)])
-@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16", "e3m12"])
 def test_physics_tile_skip_is_exact(storage, shape, x_open, monkeypatch):
     """The pure-fluid tile path against the per-site mixed path (every tile marked mixed), in
     every block order."""
@@ -239,7 +239,10 @@ def test_physics_tile_skip_is_exact(storage, shape, x_open, monkeypatch):
                                                         ("f16", 5e-5, True, True),
                                                         ("bf16", 5e-4, True, True),
                                                         ("f32", 2e-6, False, False),
-                                                        ("f16", 2e-4, False, False)])
+                                                        ("f16", 2e-4, False, False),
+                                                        ("e3m12", 1e-4, False, True),
+                                                        ("e3m12", 2e-5, True, True),
+                                                        ("e3m12", 1e-4, False, False)])
 def test_physics_reduced_storage_close_to_f64(storage, tol, shifted, x_open):
     """Populations after 40 steps within ``tol`` (absolute; populations are O(1/36)) of FP64;
     the free-stream-shifted deviations have the tighter bound.  Open and periodic x."""
@@ -261,7 +264,7 @@ def test_physics_reduced_storage_close_to_f64(storage, tol, shifted, x_open):
     assert err < tol, f"{storage}: max abs population error {err}"
 
 
-@pytest.mark.parametrize("storage,tol", [("f32", 2e-6), ("f16", 2e-4), ("bf16", 2e-3)])
+@pytest.mark.parametrize("storage,tol", [("f32", 2e-6), ("f16", 2e-4), ("bf16", 2e-3), ("e3m12", 1e-4)])
 @pytest.mark.parametrize("x_open,shape", [(True, (130, 6, 5)), (False, (130, 6, 5)),
                                           (False, (7, 3, 2))])
 def test_physics_reduced_storage_scattered_solids(storage, tol, x_open, shape):
