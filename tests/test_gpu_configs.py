@@ -71,10 +71,11 @@ def test_c2_ridged_256x128x128_reduced_storage_against_fp64():
 
 @pytest.mark.timeout(600)
 def test_c3_sponge_512x256x256_reduced_storage_against_fp64():
-    # measured on MI355X: max |df| 2.5e-6 / 2.4e-4, |dC_D|/C_D 3.5e-5 / 6.2e-4 (FP32 / FP16)
+    # measured on MI355X: max |df| 2.5e-6 / 2.4e-4 / 5.8e-5, |dC_D|/C_D 3.5e-5 / 6.2e-4 / 5.9e-5
+    # (FP32 / FP16 / e3m12)
     compare_with_f64("sponge_512x256x256", ["f32", "f16", "e3m12"], 100,
-                     pop_tol={"f32": 2e-5, "f16": 1e-3, "e3m12": 1e-3},
-                     cd_tol={"f32": 2e-4, "f16": 3e-3, "e3m12": 3e-3})
+                     pop_tol={"f32": 2e-5, "f16": 1e-3, "e3m12": 3e-4},
+                     cd_tol={"f32": 2e-4, "f16": 3e-3, "e3m12": 3e-4})
 
 
 @pytest.mark.timeout(900)
