@@ -335,8 +335,12 @@ template <> struct Store<bf16_t> {
 // format's range shows up as non-finite.
 //   encode: v = d 2^-121 / w_q puts the format's subnormal range onto FP32 denormals (kept by the
 //           kernels: float denormal mode preserved), so the FP32 bits of |v| are the code scaled by
-//           2^11: round to nearest even at bit 11 and shift;
-//   decode: the code shifted back into FP32 bits, times w_q 2^121.
+//           2^11: clamp, round to nearest with ties away from zero at bit 11, and shift -- built in
+//           the upper half of a 32-bit word, where FP32's own sign bit already sits (4 VALU
+//           operations; a tie needs the 11 dropped bits to be exactly 0x400, and rounding it away
+//           from zero biases a magnitude by 2^-13 of its last place at most);
+//   decode: the code shifted back into FP32 bits (sign-extending 16-bit load, shift, mask), times
+//           w_q 2^121.
 struct e3m12_t {
     uint16_t bits;
 };
@@ -344,21 +348,53 @@ struct e3m12_t {
 template <> struct Store<e3m12_t> {
     using compute = float;
     static constexpr bool deviation = true;
+    // largest FP32 bit pattern of |v| whose rounded code is below the NaN code's rounding range
+    static constexpr uint32_t kTop = 0x03FFFBFFu;
+    // the FP32 bits of the NaN code's value, 0x7FFF << 11
+    static constexpr uint32_t kNanBits = 0x03FFF800u;
+    // the code in the upper 16 bits of the result (the lower half is not part of it)
+    static __device__ __forceinline__ uint32_t code_hi(float d, int q)
+    {
+        // |d| through the multiply's input modifier; a NaN or infinity has bits above kTop
+        uint32_t a = __float_as_uint(__builtin_fabsf(d) * (0x1p-121f / weight<float>(q)));
+        a = a < kTop ? a : kTop;
+        uint32_t u = (a << 5) + 0x8000u; // v_lshl_add_u32
+        // hide the known zero sign bit of u, which would let the compiler move the rounding add
+        // behind the sign merge (3 operations instead of v_and_or_b32)
+        asm volatile("" : "+v"(u));
+        return (__float_as_uint(d) & 0x80000000u) | u;
+    }
+    // FP32 bits of the stored value before the scale w_q 2^121 (sign and magnitude in place)
+    static __device__ __forceinline__ uint32_t value_bits(e3m12_t x)
+    {
+        const uint32_t s = (uint32_t)(int32_t)(int16_t)x.bits;
+        return (s << 11) & 0x83FFF800u;
+    }
     static __device__ __forceinline__ e3m12_t raw_store(float d, int q)
     {
-        // |d| through the multiply's input modifier, the sign from d's own bits
-        const uint32_t a = __float_as_uint(__builtin_fabsf(d) * (0x1p-121f / weight<float>(q)));
-        uint32_t c = (a + 0x3FFu + ((a >> 11) & 1u)) >> 11;
-        c = c < 0x7FFFu ? c : 0x7FFFu;
         e3m12_t r;
-        r.bits = (uint16_t)(((__float_as_uint(d) >> 16) & 0x8000u) | c);
+        r.bits = (uint16_t)(code_hi(d, q) >> 16);
         return r;
     }
     static __device__ __forceinline__ float raw_load(e3m12_t x, int q)
     {
-        const uint32_t b = x.bits;
-        const float v = __uint_as_float(((b & 0x8000u) << 16) | ((b & 0x7FFFu) << 11)) * (weight<float>(q) * 0x1p121f);
-        return (b & 0x7FFFu) == 0x7FFFu ? __builtin_nanf("") : v;
+        const uint32_t b = value_bits(x);
+        const float v = __uint_as_float(b) * (weight<float>(q) * 0x1p121f);
+        return (b & 0x7FFFFFFFu) == kNanBits ? __builtin_nanf("") : v;
+    }
+    // The step kernel's load: the NaN code is not tested per value; the largest magnitude of a
+    // site's loads is kept in `top` and tested once (nan_if_top), so a NaN code among them makes
+    // the site's update NaN -- also one it loads without using (the own populations of a mixed
+    // tile's site, the row-end lanes' clamped pulls), which a NaN state does not mind.
+    static __device__ __forceinline__ float step_load(e3m12_t x, int q, float &top)
+    {
+        const float h = __uint_as_float(value_bits(x));
+        top = __builtin_fmaxf(top, __builtin_fabsf(h));
+        return h * (weight<float>(q) * 0x1p121f);
+    }
+    static __device__ __forceinline__ float nan_if_top(float top, float v)
+    {
+        return top >= __uint_as_float(kNanBits) ? __builtin_nanf("") : v;
     }
     static __device__ __forceinline__ float load(e3m12_t v, float ref, int q) { return ref + raw_load(v, q); }
     static __device__ __forceinline__ e3m12_t store(float f, float ref, int q) { return raw_store(f - ref, q); }

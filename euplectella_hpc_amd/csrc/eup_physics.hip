@@ -60,10 +60,20 @@ template <typename S> struct XArgs {
 // computes on full values in the reference's evaluation order (collide).
 template <typename S> constexpr bool kOnStored = std::is_same_v<typename Store<S>::compute, float>;
 
-template <typename S>
-__device__ __forceinline__ typename Store<S>::compute ld(const XArgs<S> &a, const S *p, int q)
+// e3m12 loads leave the NaN test of the stored codes to one test per site (`top`, nan_if_top)
+template <typename S> constexpr bool kTopTest = std::is_same_v<S, e3m12_t>;
+
+// a stored deviation as the kernel computes on it (FP32-compute storages)
+template <typename S> __device__ __forceinline__ float step_raw(S v, int q, float &top)
 {
-    if constexpr (kOnStored<S>) return Store<S>::raw_load(*p, q);
+    if constexpr (kTopTest<S>) return Store<S>::step_load(v, q, top);
+    else return Store<S>::raw_load(v, q);
+}
+
+template <typename S>
+__device__ __forceinline__ typename Store<S>::compute ld(const XArgs<S> &a, const S *p, int q, float &top)
+{
+    if constexpr (kOnStored<S>) return step_raw<S>(*p, q, top);
     else return Store<S>::load(*p, a.ref.r[q], q);
 }
 
@@ -128,7 +138,8 @@ __device__ __forceinline__ const S *at(const char *p, int64_t bytes, uint32_t of
 // solid) the site's own opposite population.  All 19 loads are issued back to back; the sites
 // whose source lies outside the lattice in x are fixed afterwards (x_edges).
 template <typename S, bool MIXED, int Q>
-__device__ __forceinline__ void pull(const XArgs<S> &a, const Wave &w, uint32_t lm, typename Store<S>::compute *x)
+__device__ __forceinline__ void pull(const XArgs<S> &a, const Wave &w, uint32_t lm, typename Store<S>::compute *x,
+                                     float &top)
 {
     constexpr int cx = Dir<Q>::cx, cy = Dir<Q>::cy, cz = Dir<Q>::cz, op = Dir<Q>::opp;
     constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
@@ -149,8 +160,8 @@ __device__ __forceinline__ void pull(const XArgs<S> &a, const Wave &w, uint32_t 
             }
         }
         // (the bounce-back path is taken by FP64 only: the other storages use the at-once form)
-        if constexpr (MIXED && Store<S>::deviation) x[Q] = Store<S>::raw_load(*p, Q) + corr;
-        else x[Q] = Store<S>::raw_load(*p, Q);
+        if constexpr (MIXED && Store<S>::deviation) x[Q] = step_raw<S>(*p, Q, top) + corr;
+        else x[Q] = step_raw<S>(*p, Q, top);
     } else {
         float ref = a.ref.r[Q];
         if constexpr (MIXED) {
@@ -166,9 +177,9 @@ __device__ __forceinline__ void pull(const XArgs<S> &a, const Wave &w, uint32_t 
 
 template <typename S, bool MIXED, int... Qs>
 __device__ __forceinline__ void pull_all(QSeq<Qs...>, const XArgs<S> &a, const Wave &w, uint32_t lm,
-                                         typename Store<S>::compute *x)
+                                         typename Store<S>::compute *x, float &top)
 {
-    (pull<S, MIXED, Qs>(a, w, lm, x), ...);
+    (pull<S, MIXED, Qs>(a, w, lm, x, top), ...);
 }
 
 // Second pass, populations with c_x = +1 (Q = 0..4) at the site x = 0 and c_x = -1 (Q = 9..13) at
@@ -176,7 +187,8 @@ __device__ __forceinline__ void pull_all(QSeq<Qs...>, const XArgs<S> &a, const W
 // the far end of the row (unless the bounce-back of the first pass applies).  Only the first and
 // the last tile of a row reach this code (wave-uniform tests).
 template <typename S, bool XOPEN>
-__device__ __forceinline__ void x_edges(const XArgs<S> &a, const Wave &w, uint32_t lm, typename Store<S>::compute *x)
+__device__ __forceinline__ void x_edges(const XArgs<S> &a, const Wave &w, uint32_t lm, typename Store<S>::compute *x,
+                                        float &top)
 {
     using R = typename Store<S>::compute;
     constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
@@ -194,7 +206,7 @@ __device__ __forceinline__ void x_edges(const XArgs<S> &a, const Wave &w, uint32
                     const char *rb = w.row[1 - cys[Q]][1 - czs[Q]];
                     x[Q] = ld<S>(a, at<S>(rb, (int64_t)Q * kTile * sizeof(S) + (int64_t)(a.tpr - 1) * TB,
                                           (uint32_t)((a.l - 1) & 63) * (uint32_t)sizeof(S)),
-                                 Q);
+                                 Q, top);
                 }
             }
         }
@@ -212,7 +224,7 @@ __device__ __forceinline__ void x_edges(const XArgs<S> &a, const Wave &w, uint32
             } else {
                 if (edge && !((lm >> Q) & 1u)) {
                     const char *rb = w.row[1 - cys[Q]][1 - czs[Q]];
-                    x[Q] = ld<S>(a, at<S>(rb, (int64_t)Q * kTile * sizeof(S) - (int64_t)w.ti * TB, 0u), Q);
+                    x[Q] = ld<S>(a, at<S>(rb, (int64_t)Q * kTile * sizeof(S) - (int64_t)w.ti * TB, 0u), Q, top);
                 }
             }
         }
@@ -288,10 +300,13 @@ __device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, const
         if (lm & 0x80000000u) return; // solid site: frozen
     }
     R x[19];
-    pull_all<S, MIXED>(Q18{}, a, w, lm, x);
-    x[18] = ld<S>(a, at<S>(w.row[1][1], 18 * kTile * (int64_t)sizeof(S), w.o0), 18);
-    x_edges<S, XOPEN>(a, w, lm, x);
+    float top = 0.0f;
+    pull_all<S, MIXED>(Q18{}, a, w, lm, x, top);
+    x[18] = ld<S>(a, at<S>(w.row[1][1], 18 * kTile * (int64_t)sizeof(S), w.o0), 18, top);
+    x_edges<S, XOPEN>(a, w, lm, x, top);
     if (w.i >= a.l) return; // padding lanes of a partial last tile
+    // a NaN code among the loads makes the site's density, and so its whole update, NaN
+    if constexpr (kTopTest<S>) x[18] = Store<S>::nan_if_top(top, x[18]);
     if constexpr (kOnStored<S>) collide_dev<Store<S>::deviation>(x, a.bgk);
     else collide<R>(x, x, a.omega, a.fgrad);
     put_all<S>(Q19{}, a, reinterpret_cast<char *>(a.B) + (int64_t)t * TB, w.o0, x);
@@ -306,15 +321,16 @@ __device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, const
 // (512 x 256 x 256 sponge, 8 % mixed tiles): FP32 +3.3 %, FP16 +2.4 %, FP64 -0.6 % (whose 100
 // registers per lane leave no room), so FP64 keeps update_tile<MIXED>.
 template <typename S, int Q>
-__device__ __forceinline__ void own_val(const XArgs<S> &a, const Wave &w, typename Store<S>::compute *o)
+__device__ __forceinline__ void own_val(const XArgs<S> &a, const Wave &w, typename Store<S>::compute *o, float &top)
 {
-    o[Q] = ld<S>(a, at<S>(w.row[1][1], (int64_t)Q * kTile * sizeof(S), w.o0), Q);
+    o[Q] = ld<S>(a, at<S>(w.row[1][1], (int64_t)Q * kTile * sizeof(S), w.o0), Q, top);
 }
 
 template <typename S, int... Qs>
-__device__ __forceinline__ void own_all(QSeq<Qs...>, const XArgs<S> &a, const Wave &w, typename Store<S>::compute *o)
+__device__ __forceinline__ void own_all(QSeq<Qs...>, const XArgs<S> &a, const Wave &w, typename Store<S>::compute *o,
+                                        float &top)
 {
-    (own_val<S, Qs>(a, w, o), ...);
+    (own_val<S, Qs>(a, w, o, top), ...);
 }
 
 template <typename S, int Q>
@@ -342,13 +358,15 @@ __device__ __forceinline__ void update_mixed_at_once(const XArgs<S> &a, uint32_t
     constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
     const uint32_t lm = *lmp;
     R x[19], own[18];
-    pull_all<S, false>(Q18{}, a, w, 0u, x);
-    x[18] = ld<S>(a, at<S>(w.row[1][1], 18 * kTile * (int64_t)sizeof(S), w.o0), 18);
-    own_all<S>(Q18{}, a, w, own);
+    float top = 0.0f;
+    pull_all<S, false>(Q18{}, a, w, 0u, x, top);
+    x[18] = ld<S>(a, at<S>(w.row[1][1], 18 * kTile * (int64_t)sizeof(S), w.o0), 18, top);
+    own_all<S>(Q18{}, a, w, own, top);
     if (lm & 0x80000000u) return; // solid site: frozen
     choose_all<S>(Q18{}, a, lm, own, x);
-    x_edges<S, XOPEN>(a, w, lm, x);
+    x_edges<S, XOPEN>(a, w, lm, x, top);
     if (w.i >= a.l) return;
+    if constexpr (kTopTest<S>) x[18] = Store<S>::nan_if_top(top, x[18]);
     if constexpr (kOnStored<S>) collide_dev<Store<S>::deviation>(x, a.bgk);
     else collide<R>(x, x, a.omega, a.fgrad);
     put_all<S>(Q19{}, a, reinterpret_cast<char *>(a.B) + (int64_t)t * TB, w.o0, x);
