@@ -6,7 +6,8 @@ population storage; the same run with FP16 deviation storage is reported beside 
 and with the e3m12 16-bit deviation storage (``also``).  Inputs are synthetic (seeded generators),
 resident in HBM before the timed region.  The metric's qualifier, |dC_D|/C_D <= 1e-3 against FP64,
 is measured for each storage in the same run (``accuracy``; ``qualified_16bit`` names the 16-bit
-storage that meets it at the workload's own tau, with its speed-up over FP32).
+storage that meets it at the workload's own tau, with its speed-up over FP32), beside the floor of
+each window: FP64 against FP64 started from its own state rounded once to FP32.
 
 A step is one fused collide+stream launch over the whole lattice.  MLUPS follows the reference's
 definition (all l*m*n sites per second, CODE/SRC/finalize.F90:107-108).  The roofline figure uses
@@ -99,7 +100,7 @@ def parse_storage(spec):
     return (spec[:-1], True) if spec in ("f16s", "bf16s", "e3m12s") else (spec, False)
 
 
-def build_lattice(wl, storage, mask, chain=True, mode="physics"):
+def build_lattice(wl, storage, mask, chain=True, mode="physics", noise_seed=None):
     from euplectella_hpc_amd.lib import Lattice, svisc_from_tau
 
     l, m, n = wl["l"], wl["m"], wl["n"]
@@ -108,7 +109,7 @@ def build_lattice(wl, storage, mask, chain=True, mode="physics"):
                   storage=st, shifted=shifted, x_open=wl.get("x_open", True))
     lat.set_mask(mask)
     # u = (u_in, +-1e-4 seeded noise, 0) at every site, generated on the device
-    lat.init_equilibrium(wl["u_in"], 0.0, 0.0, 1e-4, wl["seed"])
+    lat.init_equilibrium(wl["u_in"], 0.0, 0.0, 1e-4, wl["seed"] if noise_seed is None else noise_seed)
     for tau, steps in (wl.get("chain", ()) if chain else ()):
         if tau <= wl["tau"]:
             break  # the chain only runs stages of higher viscosity than the target
@@ -307,7 +308,11 @@ def copy_probe(mib, reps=10):
 ACCURACY_BOUND = 1e-3  # |dC_D|/C_D of the time-mean drag against FP64 (the metric's qualifier)
 
 
-def accuracy(wl, storages, steps, every, window=0.5, mask=None, survive=0, quiet=False):
+FLOOR = "f64_rounded_start"  # FP64 from the same start-up, every population rounded to FP32 once
+
+
+def accuracy(wl, storages, steps, every, window=0.5, mask=None, survive=0, quiet=False, floor=False,
+             noise_seed=None):
     """Drag coefficient and probe of reduced storage against an FP64 run of the same workload.
 
     Every storage runs the same start-up chain and then ``steps`` steps at the workload's tau.
@@ -316,6 +321,11 @@ def accuracy(wl, storages, steps, every, window=0.5, mask=None, survive=0, quiet
     last ``window`` of the run and |dC_D|/C_D against FP64, and the max probe velocity error.
     With ``survive`` > steps the FP64 run continues (checked every 50 steps) up to ``survive``
     steps after the chain and reports the last step at which it was still finite.
+    With ``floor`` one more FP64 run starts the window from the FP64 state with every population
+    rounded to FP32 once (a perturbation of 2^-24 relative, below what FP32 storage rounds every
+    step): its difference from FP64 is how far rounding-level perturbations alone carry the time-mean
+    drag of this flow in this window, the floor under which no storage error is resolved.
+    ``noise_seed`` replaces the workload's seed of the start-up noise (the geometry keeps its own).
     """
     if mask is None:
         mask = workload_mask(wl)
@@ -323,8 +333,11 @@ def accuracy(wl, storages, steps, every, window=0.5, mask=None, survive=0, quiet
     norm = 0.5 * wl["u_in"] ** 2 * wl["diameter"] * wl["n"]
     hist = {}
     survived = None
-    for st in ["f64"] + [x for x in storages if x != "f64"]:
-        lat = build_lattice(wl, st, mask)
+    runs = ["f64"] + [x for x in storages if x != "f64"] + ([FLOOR] if floor else [])
+    for st in runs:
+        lat = build_lattice(wl, "f64" if st == FLOOR else st, mask, noise_seed=noise_seed)
+        if st == FLOOR:
+            lat.set_state(lat.get_state().astype(np.float32).astype(np.float64))
         cd, pr = [], []
         t0 = time.perf_counter()
         try:
@@ -355,7 +368,8 @@ def accuracy(wl, storages, steps, every, window=0.5, mask=None, survive=0, quiet
     cut = int(len(hist["f64"][0]) * (1 - window))
     ref_cd = hist["f64"][0][cut:].mean()
     out = {"workload": wl_name(wl), "tau": wl["tau"], "u_in": wl["u_in"], "steps": steps,
-           "every": every, "window": window, "cd_f64": ref_cd,
+           "every": every, "window": window, "noise_seed": wl["seed"] if noise_seed is None else noise_seed,
+           "cd_f64": ref_cd,
            "finite_f64": bool(np.all(np.isfinite(hist["f64"][0])))}
     if survive > steps:
         out["f64_finite_through_step"] = survived
@@ -365,27 +379,41 @@ def accuracy(wl, storages, steps, every, window=0.5, mask=None, survive=0, quiet
             continue
         m = cd[cut:].mean()
         rel = abs(m - ref_cd) / abs(ref_cd)
-        out[st] = {"cd": m, "rel_dcd": rel, "met": bool(rel <= ACCURACY_BOUND),
+        out[st] = {"cd": m, "rel_dcd": rel,
                    "max_rel_dcd_inst": float(np.max(np.abs(cd - hist["f64"][0]) / abs(ref_cd))),
                    "probe_max_abs_du": float(np.max(np.abs(pr - hist["f64"][1])))}
+        if st != FLOOR:
+            out[st]["met"] = bool(rel <= ACCURACY_BOUND)
     return out
+
+
+NOISE_SEEDS = 3  # start-up noise realisations of the window at the workload's tau
 
 
 def accuracy_qualifier(wl, mask, storages):
     """The metric's |dC_D|/C_D <= 1e-3 qualifier, measured against FP64 on the benchmark's own
     geometry in two windows: 800 steps at the workload's tau after the start-up chain (with how
-    long FP64 itself stays finite there), and 10 000 steps at tau = 0.52 (the lowest viscosity of
-    the chain, where plain BGK is stable on this geometry)."""
+    long FP64 itself stays finite there), from NOISE_SEEDS realisations of the start-up noise --
+    in this transitional flow the storage's rounding, injected every step, moves the time-mean by
+    amounts that differ from one trajectory to the next, so the qualifier holds at a tau only if
+    every realisation meets it -- and 10 000 steps at tau = 0.52 (the lowest viscosity of the
+    chain, where plain BGK is stable on this geometry)."""
     out = {"criterion": f"|dC_D|/C_D <= {ACCURACY_BOUND:g} of the time-mean over the last half of the window "
                         "against FP64 physics mode (same geometry, start-up and steps; C_D from the "
                         "momentum-exchange force)",
            "windows": []}
-    out["windows"].append(accuracy(wl, storages, 800, 20, mask=mask, survive=3000, quiet=True))
+    out["windows"].append(accuracy(wl, storages, 800, 20, mask=mask, survive=3000, quiet=True, floor=True))
+    for k in range(1, NOISE_SEEDS):
+        out["windows"].append(accuracy(wl, storages, 800, 20, mask=mask, quiet=True, noise_seed=wl["seed"] + k))
     if wl["tau"] < 0.52:
-        out["windows"].append(accuracy(dict(wl, tau=0.52), storages, 10000, 50, mask=mask, quiet=True))
+        out["windows"].append(accuracy(dict(wl, tau=0.52), storages, 10000, 50, mask=mask, quiet=True,
+                                       floor=True))
+    taus = sorted({w["tau"] for w in out["windows"]})
     for st in storages:
-        met = [w["tau"] for w in out["windows"] if w[st]["met"]]
-        out[st] = {"met_at_tau": met, "met_at_workload_tau": wl["tau"] in met}
+        by_tau = {t: [w[st]["rel_dcd"] for w in out["windows"] if w["tau"] == t] for t in taus}
+        met = [t for t in taus if all(w[st]["met"] for w in out["windows"] if w["tau"] == t)]
+        out[st] = {"met_at_tau": met, "met_at_workload_tau": wl["tau"] in met,
+                   "max_rel_dcd_at_tau": {str(t): max(v) for t, v in by_tau.items()}}
     return out
 
 
@@ -535,7 +563,7 @@ def main(argv=None):
         return 0
     if args.accuracy:
         r = accuracy(WORKLOADS[args.workload], args.accuracy_storages.split(","), args.accuracy,
-                     args.every)
+                     args.every, floor=True)
         print(json.dumps(r))
         return 0
     if args.copy_probe:
@@ -641,7 +669,9 @@ def main(argv=None):
         sixteen = [(st, tgt) for st, tgt in targets[1:] if tgt is not None and storage_bytes(st) == 2]
         met = [(st, tgt) for st, tgt in sixteen if tgt.get("accuracy_met_at_workload_tau")]
         out["qualified_16bit"] = ({"storage": met[0][0], "speedup_vs_primary": met[0][1]["speedup_vs_primary"],
-                                   "rel_dcd": acc["windows"][0][met[0][0]]["rel_dcd"], "tau": wl["tau"]}
+                                   "rel_dcd": acc[met[0][0]]["max_rel_dcd_at_tau"][str(wl["tau"])],
+                                   "tau": wl["tau"], "noise_seeds": NOISE_SEEDS,
+                                   "floor_rel_dcd": acc["windows"][0][FLOOR]["rel_dcd"]}
                                   if met else None)
     print(json.dumps(out))
     return 0

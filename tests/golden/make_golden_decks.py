@@ -63,10 +63,10 @@ CASES = {
 }
 
 
-def edit_deck(text, lattice, extra=None):
+def edit_deck(text, lattice, extra=None, runlen=None):
     """The deck with lx/ly/lz and the run length replaced, lists reduced to their first value."""
     lx, ly, lz = lattice
-    fields = dict(lx=lx, ly=ly, lz=lz, **RUNLEN, **(extra or {}))
+    fields = dict(lx=lx, ly=ly, lz=lz, **(runlen or RUNLEN), **(extra or {}))
     out, seen = [], set()
     for line in text.splitlines():
         m = re.match(r"^(\s*)(\w+)(\s*=\s*)([^/]*?)(\s*/?\s*)$", line)
