@@ -44,7 +44,7 @@ template <typename S> struct PhysAcc {
     int l, m, n;
     __device__ __forceinline__ double operator()(int q, int i, int j, int k) const
     {
-        return (double)Store<S>::load(base[ix(q, i - 1, j - 1, k - 1)], ref.r[q], q);
+        return Store<S>::load64(base[ix(q, i - 1, j - 1, k - 1)], ref.r[q], q);
     }
 };
 

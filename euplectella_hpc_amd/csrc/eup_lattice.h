@@ -278,9 +278,15 @@ template <typename S> struct Store;
 // `ref` is the reference value of the population for deviation storage (w_q, or the free-stream
 // equilibrium of that population); full-value storages ignore it.  `q` is the population (0..18):
 // only the weight-scaled format (e3m12_t) uses it, as a compile-time constant at every call site.
+// load64 / store64 convert between the stored value and a full value in double (state set / get,
+// equilibrium start, diagnostics): the deviation storages form ref + d and f - ref in double, where
+// both are exact for the values they produce, so that storing what load64 returned gives back the
+// stored code (a restart through the state changes nothing).
 template <> struct Store<double> {
     using compute = double;
     static constexpr bool deviation = false;
+    static __device__ __forceinline__ double load64(double v, float, int) { return v; }
+    static __device__ __forceinline__ double store64(double f, float, int) { return f; }
     static __device__ __forceinline__ double load(double v, float, int) { return v; }
     static __device__ __forceinline__ double store(double f, float, int) { return f; }
     static __device__ __forceinline__ double raw_load(double v, int) { return v; }
@@ -290,6 +296,8 @@ template <> struct Store<double> {
 template <> struct Store<float> {
     using compute = float;
     static constexpr bool deviation = false;
+    static __device__ __forceinline__ double load64(float v, float, int) { return (double)v; }
+    static __device__ __forceinline__ float store64(double f, float, int) { return (float)f; }
     static __device__ __forceinline__ float load(float v, float, int) { return v; }
     static __device__ __forceinline__ float store(float f, float, int) { return f; }
     static __device__ __forceinline__ float raw_load(float v, int) { return v; }
@@ -305,6 +313,8 @@ template <> struct Store<_Float16> {
     // the stored deviation itself (kernels that compute on deviations)
     static __device__ __forceinline__ float raw_load(_Float16 v, int) { return (float)v; }
     static __device__ __forceinline__ _Float16 raw_store(float d, int) { return (_Float16)d; }
+    static __device__ __forceinline__ double load64(_Float16 v, float ref, int) { return (double)ref + (double)(float)v; }
+    static __device__ __forceinline__ _Float16 store64(double f, float ref, int) { return (_Float16)(float)(f - (double)ref); }
 };
 
 // deviation f - ref in bfloat16 with round-to-nearest-even
@@ -324,6 +334,8 @@ template <> struct Store<bf16_t> {
     static __device__ __forceinline__ bf16_t store(float f, float ref, int) { return from_float(f - ref); }
     static __device__ __forceinline__ float raw_load(bf16_t v, int) { return to_float(v); }
     static __device__ __forceinline__ bf16_t raw_store(float d, int) { return from_float(d); }
+    static __device__ __forceinline__ double load64(bf16_t v, float ref, int) { return (double)ref + (double)to_float(v); }
+    static __device__ __forceinline__ bf16_t store64(double f, float ref, int) { return from_float((float)(f - (double)ref)); }
 };
 
 // Deviation f_q - ref_q divided by w_q, as a 16-bit float with 1 sign, 3 exponent and 12 mantissa
@@ -392,12 +404,21 @@ template <> struct Store<e3m12_t> {
         top = __builtin_fmaxf(top, __builtin_fabsf(h));
         return h * (weight<float>(q) * 0x1p121f);
     }
+    // the stored value with the NaN code read as its magnitude: the outlet ghost plane, which the
+    // step kernel and the host's refresh compute alike (a NaN code at x = l makes the site's own
+    // update NaN through nan_if_top; the plane itself stays finite)
+    static __device__ __forceinline__ float load_magnitude(e3m12_t x, int q)
+    {
+        return __uint_as_float(value_bits(x)) * (weight<float>(q) * 0x1p121f);
+    }
     static __device__ __forceinline__ float nan_if_top(float top, float v)
     {
         return top >= __uint_as_float(kNanBits) ? __builtin_nanf("") : v;
     }
     static __device__ __forceinline__ float load(e3m12_t v, float ref, int q) { return ref + raw_load(v, q); }
     static __device__ __forceinline__ e3m12_t store(float f, float ref, int q) { return raw_store(f - ref, q); }
+    static __device__ __forceinline__ double load64(e3m12_t v, float ref, int q) { return (double)ref + (double)raw_load(v, q); }
+    static __device__ __forceinline__ e3m12_t store64(double f, float ref, int q) { return raw_store((float)(f - (double)ref), q); }
 };
 
 } // namespace eup

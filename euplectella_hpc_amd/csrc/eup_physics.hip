@@ -248,44 +248,68 @@ template <typename S> __device__ __forceinline__ void store_val(S *p, S v)
 #endif
 }
 
+// the stored value of population Q from the kernel's representation
 template <typename S, int Q>
-__device__ __forceinline__ void put(const XArgs<S> &a, char *tb, uint32_t o0, const typename Store<S>::compute *x)
+__device__ __forceinline__ void conv(const XArgs<S> &a, const typename Store<S>::compute *x, S *sv)
 {
-    S *p = reinterpret_cast<S *>(tb + (int64_t)Q * kTile * sizeof(S) + (uint64_t)o0);
-    if constexpr (kOnStored<S>) store_val<S>(p, Store<S>::raw_store(x[Q], Q));
-    else store_val<S>(p, Store<S>::store(x[Q], a.ref.r[Q], Q));
+    if constexpr (kOnStored<S>) sv[Q] = Store<S>::raw_store(x[Q], Q);
+    else sv[Q] = Store<S>::store(x[Q], a.ref.r[Q], Q);
+}
+
+// convert population Q, keep the stored value in sv and store it
+template <typename S, int Q>
+__device__ __forceinline__ void put(const XArgs<S> &a, char *tb, uint32_t o0, const typename Store<S>::compute *x,
+                                    S *sv)
+{
+    conv<S, Q>(a, x, sv);
+    store_val<S>(reinterpret_cast<S *>(tb + (int64_t)Q * kTile * sizeof(S) + (uint64_t)o0), sv[Q]);
 }
 
 template <typename S, int... Qs>
 __device__ __forceinline__ void put_all(QSeq<Qs...>, const XArgs<S> &a, char *tb, uint32_t o0,
-                                        const typename Store<S>::compute *x)
+                                        const typename Store<S>::compute *x, S *sv)
 {
-    (put<S, Qs>(a, tb, o0, x), ...);
+    (put<S, Qs>(a, tb, o0, x, sv), ...);
 }
 
 // The x = l sites write the next step's outlet ghost values (bcond.bc.F90:127-164) from their
-// post-collision populations x, in the step kernel's representation.
+// post-collision populations as stored (sv), in the step kernel's representation: the plane is a
+// function of the stored state, as the reference's ghost fill reads its stored arrays, and
+// k_outlet_refresh computes the same plane from a state the host sets.
 template <typename S>
-__device__ __forceinline__ void outlet_next(const XArgs<S> &a, const Wave &w, int j, int k,
-                                            const typename Store<S>::compute *x)
+__device__ __forceinline__ void outlet_next(const XArgs<S> &a, const Wave &w, int j, int k, const S *sv)
 {
     using R = typename Store<S>::compute;
     if (w.i != a.l - 1) return;
-    R o[5];
-    if constexpr (kOnStored<S> && Store<S>::deviation) {
-        // the outlet formula takes full values; the ghost plane holds deviations
-        R xf[19];
+    R o[5], xf[19];
+    // ref + stored deviation, as Store<S>::load forms it (bgk.r holds the same references as ref.r;
+    // the collision already keeps it in scalar registers); the identity for full-value storages
 #pragma unroll
-        for (int q = 0; q < 19; ++q) xf[q] = x[q] + a.bgk.r[q];
-        outlet_eq<R>(xf, o);
+    for (int q = 0; q < 19; ++q) {
+        if constexpr (kTopTest<S>) xf[q] = a.bgk.r[q] + Store<S>::load_magnitude(sv[q], q);
+        else if constexpr (kOnStored<S> && Store<S>::deviation) xf[q] = a.bgk.r[q] + Store<S>::raw_load(sv[q], q);
+        else xf[q] = Store<S>::raw_load(sv[q], q);
+    }
+    outlet_eq<R>(xf, o);
+    if constexpr (kOnStored<S> && Store<S>::deviation) {
+        // the ghost plane holds deviations
 #pragma unroll
         for (int v = 0; v < 5; ++v) o[v] -= a.bgk.r[9 + v];
-    } else {
-        outlet_eq<R>(x, o);
     }
     const size_t mn = (size_t)a.m * a.n;
 #pragma unroll
     for (int v = 0; v < 5; ++v) a.out_next[v * mn + j + (size_t)a.m * k] = o[v];
+}
+
+// Store the lane's post-collision populations x of tile t, converted once: the outlet (the x = l
+// sites) takes the values as stored.
+template <typename S, bool XOPEN>
+__device__ __forceinline__ void finish(const XArgs<S> &a, uint32_t t, const Wave &w, int j, int k,
+                                       const typename Store<S>::compute *x)
+{
+    S sv[19];
+    put_all<S>(Q19{}, a, reinterpret_cast<char *>(a.B) + (int64_t)t * ((int64_t)kTileElems * sizeof(S)), w.o0, x, sv);
+    if constexpr (XOPEN) outlet_next<S>(a, w, j, k, sv);
 }
 
 // Update tile t (addressing w, row j, k), one site per lane.  MIXED applies the link mask lm of the
@@ -295,7 +319,6 @@ __device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, const
                                             uint32_t lm)
 {
     using R = typename Store<S>::compute;
-    constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
     if constexpr (MIXED) {
         if (lm & 0x80000000u) return; // solid site: frozen
     }
@@ -309,8 +332,7 @@ __device__ __forceinline__ void update_tile(const XArgs<S> &a, uint32_t t, const
     if constexpr (kTopTest<S>) x[18] = Store<S>::nan_if_top(top, x[18]);
     if constexpr (kOnStored<S>) collide_dev<Store<S>::deviation>(x, a.bgk);
     else collide<R>(x, x, a.omega, a.fgrad);
-    put_all<S>(Q19{}, a, reinterpret_cast<char *>(a.B) + (int64_t)t * TB, w.o0, x);
-    if constexpr (XOPEN) outlet_next<S>(a, w, j, k, x);
+    finish<S, XOPEN>(a, t, w, j, k, x);
 }
 
 // Mixed tile with every load issued at once: the link masks, the 19 pulls from the fluid
@@ -355,7 +377,6 @@ __device__ __forceinline__ void update_mixed_at_once(const XArgs<S> &a, uint32_t
                                                      const uint32_t *lmp)
 {
     using R = typename Store<S>::compute;
-    constexpr int64_t TB = (int64_t)kTileElems * sizeof(S);
     const uint32_t lm = *lmp;
     R x[19], own[18];
     float top = 0.0f;
@@ -369,8 +390,7 @@ __device__ __forceinline__ void update_mixed_at_once(const XArgs<S> &a, uint32_t
     if constexpr (kTopTest<S>) x[18] = Store<S>::nan_if_top(top, x[18]);
     if constexpr (kOnStored<S>) collide_dev<Store<S>::deviation>(x, a.bgk);
     else collide<R>(x, x, a.omega, a.fgrad);
-    put_all<S>(Q19{}, a, reinterpret_cast<char *>(a.B) + (int64_t)t * TB, w.o0, x);
-    if constexpr (XOPEN) outlet_next<S>(a, w, j, k, x);
+    finish<S, XOPEN>(a, t, w, j, k, x);
 }
 
 template <typename S> constexpr bool kMixedAtOnce = sizeof(S) <= 4;
@@ -491,8 +511,7 @@ __global__ void k_from_staging(const double *__restrict__ stg, S *__restrict__ d
     const int64_t r = t / l;
     const int j = (int)(r % m), k = (int)(r / m);
     const double v = stg[(int64_t)(i + 1) + (int64_t)(l + 2) * ((int64_t)(j + 1) + (int64_t)(m + 2) * (k + 1))];
-    using R = typename Store<S>::compute;
-    const S sv = Store<S>::store((R)v, ref, q);
+    const S sv = Store<S>::store64(v, ref, q);
     const int64_t e = P(q, i, j, k);
     d0[e] = sv;
     d1[e] = sv;
@@ -509,7 +528,7 @@ __global__ void k_to_staging(const S *__restrict__ src, double *__restrict__ stg
     const int64_t r = t / l;
     const int j = (int)(r % m), k = (int)(r / m);
     stg[(int64_t)(i + 1) + (int64_t)(l + 2) * ((int64_t)(j + 1) + (int64_t)(m + 2) * (k + 1))] =
-        (double)Store<S>::load(src[P(q, i, j, k)], ref, q);
+        Store<S>::load64(src[P(q, i, j, k)], ref, q);
 }
 
 // counter-based uniform in [-1, 1) (splitmix64 finaliser)
@@ -542,8 +561,7 @@ __global__ void k_init_eq(S *base, PopIndex P, int l, int m, int64_t nsites, dou
     for (int q = 0; q < 19; ++q) {
         const double cu = cx[q] * ux + cy[q] * uy + cz[q] * uz;
         const double f = weight<double>(q) * (1.0 + (3.0 * cu + 1.5 * (3.0 * cu * cu - usq)));
-        using R = typename Store<S>::compute;
-        base[P(q, i, j, k)] = Store<S>::store((R)f, ref.r[q], q);
+        base[P(q, i, j, k)] = Store<S>::store64(f, ref.r[q], q);
     }
 }
 
@@ -565,7 +583,11 @@ __global__ void k_outlet_refresh(XArgs<S> a, typename Store<S>::compute *o0,
     P.tiled = 1;
     R x[19], o[5];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) x[q] = Store<S>::load(a.A[P(q, a.l - 1, j, k)], a.ref.r[q], q);
+    for (int q = 0; q < 19; ++q) {
+        // as outlet_next takes the stored values
+        if constexpr (kTopTest<S>) x[q] = a.ref.r[q] + Store<S>::load_magnitude(a.A[P(q, a.l - 1, j, k)], q);
+        else x[q] = Store<S>::load(a.A[P(q, a.l - 1, j, k)], a.ref.r[q], q);
+    }
     outlet_eq<R>(x, o);
 #pragma unroll
     for (int u = 0; u < 5; ++u) {

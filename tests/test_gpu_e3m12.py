@@ -71,7 +71,7 @@ def test_state_round_trip_equals_the_format(shifted):
     for q in range(19):
         cand = r[q].astype(np.float64) + np.where(rng.random(400_000) < 0.5, -1, 1) * \
             10.0 ** rng.uniform(-6, np.log10(3.9), 400_000) * w[q]
-        d = (cand.astype(F32) - r[q]).astype(F32)
+        d = (cand - r[q].astype(np.float64)).astype(F32)
         tie = cand[(scaled_bits(d, w[q]) & np.uint32(0x7FF)) == np.uint32(0x400)][:l]
         assert tie.size >= 20, (q, tie.size)
         pops[q, 2, 2, 1:1 + tie.size] = tie
@@ -81,8 +81,10 @@ def test_state_round_trip_equals_the_format(shifted):
         got = lat.get_state()[:, 1:-1, 1:-1, 1:-1]
     want = np.empty_like(got)
     for q in range(19):
-        d = (pops[q, 1:-1, 1:-1, 1:-1].astype(F32) - r[q]).astype(F32)
-        want[q] = (r[q] + decode(encode(d, w[q]), w[q])).astype(F32).astype(np.float64)
+        # set_state takes d = f - ref in double, rounded once to FP32; get_state returns ref + the
+        # decoded deviation in double
+        d = (pops[q, 1:-1, 1:-1, 1:-1] - r[q].astype(np.float64)).astype(F32)
+        want[q] = r[q].astype(np.float64) + decode(encode(d, w[q]), w[q]).astype(np.float64)
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
     # and the format's error bounds: 2^-13 relative on the binades, 2^-18 w_q below them
     d = got - pops[:, 1:-1, 1:-1, 1:-1]
