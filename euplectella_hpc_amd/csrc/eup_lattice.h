@@ -281,7 +281,8 @@ template <typename S> struct Store;
 // load64 / store64 convert between the stored value and a full value in double (state set / get,
 // equilibrium start, diagnostics): the deviation storages form ref + d and f - ref in double, where
 // both are exact for the values they produce, so that storing what load64 returned gives back the
-// stored code (a restart through the state changes nothing).
+// stored code (a restart through the state changes nothing) -- for every code of FP16 and e3m12,
+// and for BF16 deviations above 2^-53 of the reference (below, ref + d rounds to ref).
 template <> struct Store<double> {
     using compute = double;
     static constexpr bool deviation = false;
