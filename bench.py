@@ -387,6 +387,19 @@ def accuracy(wl, storages, steps, every, window=0.5, mask=None, survive=0, quiet
     return out
 
 
+def seed_distribution(windows, storages):
+    """|dC_D|/C_D of each storage over windows that differ only in the start-up noise: how many
+    meet the bound, and the smallest, median and largest."""
+    out = {}
+    for st in storages:
+        v = np.array([w[st]["rel_dcd"] for w in windows if st in w], dtype=np.float64)
+        if v.size == 0:
+            continue
+        out[st] = {"runs": int(v.size), "met": int(np.sum(v <= ACCURACY_BOUND)),
+                   "min": float(v.min()), "median": float(np.median(v)), "max": float(v.max())}
+    return out
+
+
 NOISE_SEEDS = 3  # start-up noise realisations of the window at the workload's tau
 
 
@@ -537,6 +550,9 @@ def main(argv=None):
     ap.add_argument("--accuracy", type=int, default=0, metavar="STEPS",
                     help="only run the C_D / probe accuracy study of f32, f16, bf16 vs f64")
     ap.add_argument("--every", type=int, default=100, help="drag sampling interval (--accuracy)")
+    ap.add_argument("--seeds", type=int, default=1,
+                    help="--accuracy: realisations of the start-up noise (seeds workload seed + 0..N-1), "
+                         "with the distribution of |dC_D|/C_D per storage")
     ap.add_argument("--accuracy-storages", default="f32,f16,f16s,bf16,bf16s,e3m12,e3m12s",
                     help="storages compared with FP64 by --accuracy")
     ap.add_argument("--no-chain", action="store_true",
@@ -562,8 +578,15 @@ def main(argv=None):
                                    not args.no_chain)))
         return 0
     if args.accuracy:
-        r = accuracy(WORKLOADS[args.workload], args.accuracy_storages.split(","), args.accuracy,
-                     args.every, floor=True)
+        wl = WORKLOADS[args.workload]
+        storages = args.accuracy_storages.split(",")
+        if args.seeds <= 1:
+            r = accuracy(wl, storages, args.accuracy, args.every, floor=True)
+        else:
+            mask = workload_mask(wl)
+            r = {"windows": [accuracy(wl, storages, args.accuracy, args.every, mask=mask, quiet=True,
+                                      floor=True, noise_seed=wl["seed"] + k) for k in range(args.seeds)]}
+            r["distribution"] = seed_distribution(r["windows"], storages + [FLOOR])
         print(json.dumps(r))
         return 0
     if args.copy_probe:

@@ -37,3 +37,14 @@ def test_cpu_baseline_leg_runs_the_reference():
     mask = geometry.cylinder(24, 12, 8, 2, (6, 6.5))
     r = refrun.time_reference(24, 12, 8, (2 * 0.6 - 1) / 6, 0.05, 5, mask, threads=2)
     assert r["mlups"] > 0 and r["threads"] == 2
+
+
+def test_seed_distribution_counts_and_orders():
+    sys.path.insert(0, ROOT)
+    import bench
+
+    windows = [{"f16": {"rel_dcd": v}, "f32": {"rel_dcd": 1e-5}} for v in (2e-3, 5e-4, 1e-3, 8e-3)]
+    d = bench.seed_distribution(windows, ["f16", "f32", bench.FLOOR])
+    assert d["f16"] == {"runs": 4, "met": 2, "min": 5e-4, "median": 1.5e-3, "max": 8e-3}
+    assert d["f32"]["met"] == 4 and d["f32"]["runs"] == 4
+    assert bench.FLOOR not in d  # no window held it
