@@ -22,10 +22,10 @@ WRITE_SIZE counters gave for this workload and storage (calibrated on the copy s
 is read from profiles/, null when no profile of this configuration is committed).
 
 CPU baseline: the reference's own OpenMP executable (built from its sources by
-oracle/Makefile.ref) on the whole 512 x 256 x 256 sponge, 50 steps (a slab of 64 z planes for
-lattices whose FP64 arrays exceed 16 GB), its own MLUPS report, on every CPU this process may use
-(OMP_NUM_THREADS, else the CPU affinity and the cgroup quota); the host's core counts are
-recorded beside it.  Without the reference build the oracle's C restatement of the serial step
+oracle/Makefile.ref) on the whole 512 x 256 x 256 sponge, 50 steps (the thickest slab of at most
+64 z planes that fits in 16 GB for lattices whose FP64 arrays exceed that), its own MLUPS report,
+on every CPU this process may use (OMP_NUM_THREADS, else the CPU affinity and the cgroup quota);
+the host's core counts are recorded beside it.  Without the reference build the oracle's C restatement of the serial step
 is timed instead (kind "port", one thread).
 """
 import argparse
@@ -253,7 +253,8 @@ def cpu_baseline_port(wl, mask=None, nz=16, steps=4):
 def cpu_baseline(wl, steps=50, nz_slab=64, mask=None):
     """The reference's OpenMP path timed on a bounded sample of the workload: the whole lattice
     when the reference's FP64 arrays fit in 16 GB of host memory (about 10 s of CPU work for the
-    512 x 256 x 256 sponge at 50 steps), else a slab of ``nz_slab`` z planes."""
+    512 x 256 x 256 sponge at 50 steps), else the thickest slab of at most ``nz_slab`` z planes
+    that fits."""
     from euplectella_hpc_amd.lib import svisc_from_tau
     from oracle import refrun
 
@@ -268,9 +269,12 @@ def cpu_baseline(wl, steps=50, nz_slab=64, mask=None):
         return 38 * (l + 2) * (m + 2) * (nz + 2) * 8
     nz, what = n, f"the whole {l}x{m}x{n} lattice of the workload"
     if footprint(n) > 16e9:
-        nz, what = nz_slab, f"a {l}x{m}x{nz_slab} slab of the {l}x{m}x{n} workload"
+        nz = nz_slab
+        while nz > 8 and footprint(nz) > 16e9:
+            nz -= 8
         if footprint(nz) > 16e9:
             return None  # the FP64 reference would not fit a bounded sample of this workload
+        what = f"a {l}x{m}x{nz} slab of the {l}x{m}x{n} workload"
     if nz != n or mask is None:
         mask = workload_mask(wl, nz)
     threads = cpu_threads(host)
@@ -306,6 +310,9 @@ def copy_probe(mib, reps=10):
 
 
 ACCURACY_BOUND = 1e-3  # |dC_D|/C_D of the time-mean drag against FP64 (the metric's qualifier)
+# the qualifier's FP64 runs are skipped for lattices whose FP64 A-B state exceeds this (the
+# 1536 x 768 x 768 sponge needs 275 GB, and each window would run its start-up in FP64)
+ACCURACY_MAX_F64_BYTES = 64e9
 
 
 FLOOR = "f64_rounded_start"  # FP64 from the same start-up, every population rounded to FP32 once
@@ -678,7 +685,11 @@ def main(argv=None):
                                           "frac": round(r["frac"], 4),
                                           "speedup_vs_primary": round(r["mlups"] / prim["mlups"], 3),
                                           "finite_after_run": r["finite"]}
-    if args.mode == "physics" and not args.no_accuracy:
+    f64_bytes = 2 * 19 * 8 * wl["l"] * wl["m"] * wl["n"]
+    if args.mode == "physics" and not args.no_accuracy and f64_bytes > ACCURACY_MAX_F64_BYTES:
+        out["accuracy"] = {"skipped": f"the FP64 state of this lattice takes {f64_bytes / 1e9:.0f} GB "
+                                      f"(above {ACCURACY_MAX_F64_BYTES / 1e9:.0f} GB)"}
+    elif args.mode == "physics" and not args.no_accuracy:
         acc = accuracy_qualifier(wl, mask, [x for x in [args.storage, args.secondary] + list(also)
                                             if x and x != "f64"])
         out["accuracy"] = _rounded(acc)
