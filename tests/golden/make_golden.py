@@ -14,6 +14,7 @@ on inputs written by this repository's seeded generators, and records:
 
 Usage (in a container where /root/reference is present):
     make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+    python tests/golden/make_golden.py c3_full   # the full-size sponge (about 10 minutes, 25 GB)
 """
 import hashlib
 import json
@@ -59,6 +60,13 @@ CASES = {
 }
 
 
+# the benchmark's 512 x 256 x 256 sponge at full size: the reference's serial step (parity mode)
+# and the oracle's physics step (physics mode FP64) after FULL_STEPS steps from the impulsive start
+FULL_C3 = dict(l=512, m=256, n=256, geom=("sponge", dict(radius=48, centre=(128, 128), cell=16,
+                                                          thickness=3.0, seed=2026)),
+               tau=0.505, u=0.06, seed=2026, steps=100, saves=(50, 100))
+
+
 def sha(arr):
     return hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest()
 
@@ -71,8 +79,33 @@ def make_inputs(case):
     return mask, pops
 
 
+def make_c3_full():
+    """Full-size sponge: SHA-256 of the reference's saved states (ghosted, parity mode) and of the
+    oracle's physics-mode interior, from the same seeded start."""
+    from oracle import oracle as orc
+
+    case = FULL_C3
+    l, m, n = case["l"], case["m"], case["n"]
+    mask, pops = make_inputs(case)
+    svisc = (2.0 * case["tau"] - 1.0) / 6.0
+    res = refrun.run_reference(l, m, n, svisc, case["u"], case["steps"], mask, pops, variant="nodrag",
+                               itsave=int(np.gcd.reduce(np.array(case["saves"]))))
+    rec = {"l": l, "m": m, "n": n, "geometry": [case["geom"][0], case["geom"][1]],
+           "tau": case["tau"], "svisc": svisc, "u_in": case["u"], "seed": case["seed"],
+           "steps": case["steps"], "nsolid": int(mask.sum()),
+           "mask_sha256": sha(mask.astype(np.int32)), "init_sha256": sha(pops),
+           "state_sha256": {str(t): sha(res["saves"][t]) for t in case["saves"]}}
+    del res
+    orc.physics_steps(pops, mask, orc.omega_from_svisc(svisc), case["u"], case["steps"])
+    rec["physics_interior_sha256"] = sha(pops[:, 1:-1, 1:-1, 1:-1])
+    print("c3_full done", rec["state_sha256"], rec["physics_interior_sha256"], flush=True)
+    return rec
+
+
 def main(names=None):
     index = {}
+    if names and "c3_full" in names:
+        index["c3_full"] = make_c3_full()
     for name, case in CASES.items():
         if names and name not in names:
             continue

@@ -70,6 +70,37 @@ def test_parity_state_hashes(golden, name, remap, monkeypatch):
             assert sha(got) == rec["state_sha256"][str(t)], f"{name}: state differs at step {t}"
 
 
+@pytest.mark.timeout(900)
+def test_full_size_sponge_parity_equals_the_reference_and_physics_fp64_the_oracle(golden):
+    """The benchmark's 512 x 256 x 256 sponge at full size, from the seeded impulsive start: parity
+    mode's ghosted state equals the reference executable's save dumps after 50 and 100 steps, and
+    physics mode in FP64 equals the oracle's physics step on every interior site after 100 steps,
+    bit for bit (SHA-256 of the states; tests/golden/make_golden.py c3_full)."""
+    from make_golden import FULL_C3
+
+    rec = golden["c3_full"]
+    l, m, n = rec["l"], rec["m"], rec["n"]
+    announce(f"full-size sponge {l}x{m}x{n}: generating the mask and the start")
+    mask, pops = make_inputs(FULL_C3)
+    assert sha(mask.astype(np.int32)) == rec["mask_sha256"] and sha(pops) == rec["init_sha256"]
+    announce("full-size sponge: parity mode, 50 + 50 steps")
+    with lattice((l, m, n), svisc=rec["svisc"], u_inflow=rec["u_in"], mode="parity", storage="f64") as lat:
+        lat.set_mask(mask)
+        lat.set_state(pops)
+        done = 0
+        for t in sorted(int(x) for x in rec["state_sha256"]):
+            lat.step(t - done)
+            done = t
+            assert sha(lat.get_state()) == rec["state_sha256"][str(t)], f"parity state differs at step {t}"
+    announce("full-size sponge: physics mode FP64, 100 steps")
+    with lattice((l, m, n), svisc=rec["svisc"], u_inflow=rec["u_in"], mode="physics", storage="f64") as lat:
+        lat.set_mask(mask)
+        lat.set_state(pops)
+        lat.step(rec["steps"])
+        got = lat.get_state()
+    assert sha(got[:, 1:-1, 1:-1, 1:-1]) == rec["physics_interior_sha256"]
+
+
 @pytest.mark.parametrize("remap", ["0", "1", "2"])
 @pytest.mark.parametrize("shape", [(320, 12, 10), (512, 8, 6)])
 def test_parity_block_orders_equal_oracle(shape, remap, monkeypatch):
