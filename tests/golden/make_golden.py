@@ -14,7 +14,7 @@ on inputs written by this repository's seeded generators, and records:
 
 Usage (in a container where /root/reference is present):
     make -f oracle/Makefile.ref && python tests/golden/make_golden.py
-    python tests/golden/make_golden.py c3_full   # the full-size sponge (about 10 minutes, 25 GB)
+    python tests/golden/make_golden.py c3_full c2_full   # full sizes (about 35 minutes, 25 GB)
 """
 import hashlib
 import json
@@ -60,11 +60,19 @@ CASES = {
 }
 
 
-# the benchmark's 512 x 256 x 256 sponge at full size: the reference's serial step (parity mode)
-# and the oracle's physics step (physics mode FP64) after FULL_STEPS steps from the impulsive start
-FULL_C3 = dict(l=512, m=256, n=256, geom=("sponge", dict(radius=48, centre=(128, 128), cell=16,
-                                                          thickness=3.0, seed=2026)),
-               tau=0.505, u=0.06, seed=2026, steps=100, saves=(50, 100))
+# the benchmark configurations at full size: the reference's serial step (parity mode) and the
+# oracle's physics step (physics mode FP64) from the seeded impulsive start
+FULL = {
+    # reticular sponge 512 x 256 x 256 (the benchmark's workload)
+    "c3_full": dict(l=512, m=256, n=256, geom=("sponge", dict(radius=48, centre=(128, 128), cell=16,
+                                                               thickness=3.0, seed=2026)),
+                    tau=0.505, u=0.06, seed=2026, steps=100, saves=(50, 100)),
+    # ridged cylinder 256 x 128 x 128
+    "c2_full": dict(l=256, m=128, n=128,
+                    geom=("ridged_cylinder", dict(radius=12, centre=(64, 64), n_ridges=8, width=3)),
+                    tau=0.52, u=0.08, seed=2026, steps=200, saves=(100, 200)),
+}
+FULL_C3 = FULL["c3_full"]
 
 
 def sha(arr):
@@ -79,12 +87,12 @@ def make_inputs(case):
     return mask, pops
 
 
-def make_c3_full():
-    """Full-size sponge: SHA-256 of the reference's saved states (ghosted, parity mode) and of the
-    oracle's physics-mode interior, from the same seeded start."""
+def make_full(name):
+    """A configuration at full size: SHA-256 of the reference's saved states (ghosted, parity mode)
+    and of the oracle's physics-mode interior, from the same seeded start."""
     from oracle import oracle as orc
 
-    case = FULL_C3
+    case = FULL[name]
     l, m, n = case["l"], case["m"], case["n"]
     mask, pops = make_inputs(case)
     svisc = (2.0 * case["tau"] - 1.0) / 6.0
@@ -98,14 +106,15 @@ def make_c3_full():
     del res
     orc.physics_steps(pops, mask, orc.omega_from_svisc(svisc), case["u"], case["steps"])
     rec["physics_interior_sha256"] = sha(pops[:, 1:-1, 1:-1, 1:-1])
-    print("c3_full done", rec["state_sha256"], rec["physics_interior_sha256"], flush=True)
+    print(name, "done", rec["state_sha256"], rec["physics_interior_sha256"], flush=True)
     return rec
 
 
 def main(names=None):
     index = {}
-    if names and "c3_full" in names:
-        index["c3_full"] = make_c3_full()
+    for name in FULL:
+        if names and name in names:
+            index[name] = make_full(name)
     for name, case in CASES.items():
         if names and name not in names:
             continue

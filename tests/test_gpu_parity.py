@@ -71,19 +71,21 @@ def test_parity_state_hashes(golden, name, remap, monkeypatch):
 
 
 @pytest.mark.timeout(900)
-def test_full_size_sponge_parity_equals_the_reference_and_physics_fp64_the_oracle(golden):
-    """The benchmark's 512 x 256 x 256 sponge at full size, from the seeded impulsive start: parity
-    mode's ghosted state equals the reference executable's save dumps after 50 and 100 steps, and
-    physics mode in FP64 equals the oracle's physics step on every interior site after 100 steps,
-    bit for bit (SHA-256 of the states; tests/golden/make_golden.py c3_full)."""
-    from make_golden import FULL_C3
+@pytest.mark.parametrize("name", ["c2_full", "c3_full"])
+def test_full_size_parity_equals_the_reference_and_physics_fp64_the_oracle(golden, name):
+    """The benchmark configurations at full size (ridged cylinder 256 x 128 x 128, 200 steps; the
+    512 x 256 x 256 sponge, 100 steps), from the seeded impulsive start: parity mode's ghosted state
+    equals the reference executable's save dumps at both save points, and physics mode in FP64
+    equals the oracle's physics step on every interior site at the end, bit for bit (SHA-256 of the
+    states; tests/golden/make_golden.py c2_full c3_full)."""
+    from make_golden import FULL
 
-    rec = golden["c3_full"]
+    rec = golden[name]
     l, m, n = rec["l"], rec["m"], rec["n"]
-    announce(f"full-size sponge {l}x{m}x{n}: generating the mask and the start")
-    mask, pops = make_inputs(FULL_C3)
+    announce(f"{name} {l}x{m}x{n}: generating the mask and the start")
+    mask, pops = make_inputs(FULL[name])
     assert sha(mask.astype(np.int32)) == rec["mask_sha256"] and sha(pops) == rec["init_sha256"]
-    announce("full-size sponge: parity mode, 50 + 50 steps")
+    announce(f"{name}: parity mode, saves at {sorted(int(x) for x in rec['state_sha256'])}")
     with lattice((l, m, n), svisc=rec["svisc"], u_inflow=rec["u_in"], mode="parity", storage="f64") as lat:
         lat.set_mask(mask)
         lat.set_state(pops)
@@ -92,7 +94,7 @@ def test_full_size_sponge_parity_equals_the_reference_and_physics_fp64_the_oracl
             lat.step(t - done)
             done = t
             assert sha(lat.get_state()) == rec["state_sha256"][str(t)], f"parity state differs at step {t}"
-    announce("full-size sponge: physics mode FP64, 100 steps")
+    announce(f"{name}: physics mode FP64, {rec['steps']} steps")
     with lattice((l, m, n), svisc=rec["svisc"], u_inflow=rec["u_in"], mode="physics", storage="f64") as lat:
         lat.set_mask(mask)
         lat.set_state(pops)
