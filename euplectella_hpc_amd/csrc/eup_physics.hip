@@ -395,11 +395,9 @@ __device__ __forceinline__ void update_mixed_at_once(const XArgs<S> &a, uint32_t
 
 template <typename S> constexpr bool kMixedAtOnce = sizeof(S) <= 4;
 
-// No register target beyond the block size: asking the compiler for 8 wavefronts per SIMD (64
-// registers) makes it interleave the 19 loads with their uses and costs the 2-byte storages
-// 7-13 % on MI355X, although it removes only one register.
+// One step of one tile per wavefront (the body of k_fused).
 template <typename S, bool XOPEN>
-__global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
+__device__ __forceinline__ void fused_step(const XArgs<S> &a)
 {
     // Block order (chosen by fused_block_order).  Hardware block b runs on XCD b % 8.  Order 0:
     // the eight XCDs sweep the lattice together in dispatch order.  Order 1: each XCD gets a
@@ -438,6 +436,27 @@ __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
         const uint32_t lm = a.link[(size_t)code * kTile + (threadIdx.x & 63)];
         update_tile<S, XOPEN, true>(a, t, w, j, k, lm);
     }
+}
+
+// No register target beyond the block size: asking the compiler for 8 wavefronts per SIMD (64
+// registers) makes it interleave the 19 loads with their uses and costs the 2-byte storages
+// 7-13 % on MI355X, although it removes only one register.
+template <typename S, bool XOPEN>
+__global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
+{
+    fused_step<S, XOPEN>(a);
+}
+
+// e3m12 with open x faces, with a budget of 96 scalar registers (101 without): still seven
+// wavefronts per SIMD (its 70 vector registers decide), and the schedule the compiler builds under
+// the budget ran the 512 x 256 x 256 sponge step in 0.431 / 0.427 ms instead of 0.442 / 0.438 ms
+// on two MI355X boxes, level with FP16 (0.425 ms).  The same budget costs
+// FP16 and BF16 3 % and 7 %: there it lets an eighth wavefront in, whose extra 19 loads in flight
+// per lane disorder the memory request stream.
+template <>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(96))) void k_fused<e3m12_t, true>(XArgs<e3m12_t> a)
+{
+    fused_step<e3m12_t, true>(a);
 }
 
 // ---------------------------------------------------------------------------------------------
