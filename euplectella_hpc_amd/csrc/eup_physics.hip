@@ -450,9 +450,9 @@ __global__ __launch_bounds__(kBlock) void k_fused(XArgs<S> a)
 // e3m12 with open x faces, with a budget of 96 scalar registers (101 without): still seven
 // wavefronts per SIMD (its 70 vector registers decide), and the schedule the compiler builds under
 // the budget ran the 512 x 256 x 256 sponge step in 0.431 / 0.427 ms instead of 0.442 / 0.438 ms
-// on two MI355X boxes, level with FP16 (0.425 ms).  The same budget costs
-// FP16 and BF16 3 % and 7 %: there it lets an eighth wavefront in, whose extra 19 loads in flight
-// per lane disorder the memory request stream.
+// on two MI355X boxes, level with FP16 (0.425 ms).  The same budget costs FP16 and BF16 3 % and
+// 7 %: there it lets an eighth wavefront in, whose extra 19 loads in flight per lane disorder the
+// memory request stream.
 template <>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(96))) void k_fused<e3m12_t, true>(XArgs<e3m12_t> a)
 {
@@ -460,8 +460,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(96))) void k
 }
 
 // FP64 with open x faces, compiled for the three wavefronts per SIMD that its block cap allows:
-// 112 registers instead of 94, and 1.624 instead of 1.637 ms per 512 x 256 x 256 sponge step on
-// MI355X (the parity stream-collide under the same hint: no change).
+// 112 registers instead of 94 (so at most four wavefronts where the cap is lifted), and 1.624
+// instead of 1.637 ms per 512 x 256 x 256 sponge step on MI355X (the parity stream-collide under
+// the same hint: no change).
 template <>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 3))) void k_fused<double, true>(XArgs<double> a)
 {
