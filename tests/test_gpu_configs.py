@@ -120,3 +120,28 @@ def test_c5_sponge_1536x768x768_fp32_and_fp16_at_full_size():
     announce(f"C5 FP16 vs FP32 at {len(sites)} sites: max |du| = {du:.3e}, max |drho| = {drho:.3e}")
     # measured on MI355X: 4.5e-5 / 8.4e-5
     assert du < 5e-4 and drho < 5e-4
+
+
+@pytest.mark.timeout(300)
+def test_accuracy_window_on_the_cylinder():
+    """bench.py's accuracy window on the C1 cylinder (128x64x64, tau 0.6), 400 steps from the
+    seeded start, in two noise realisations: FP64 stays finite, the floor (FP64 from its
+    FP32-rounded start) is far below FP32's error, FP32 and FP16 meet the bound, and the
+    distribution summary counts them."""
+    wl = bench.WORKLOADS["cylinder_128x64x64"]
+    mask = bench.workload_mask(wl)
+    windows = []
+    for k in range(2):
+        announce(f"C1 accuracy window, 400 steps, noise seed {wl['seed'] + k}")
+        windows.append(bench.accuracy(wl, ["f32", "f16"], 400, 20, mask=mask, quiet=True, floor=True,
+                                      noise_seed=wl["seed"] + k))
+    for w in windows:
+        announce(f"  seed {w['noise_seed']}: " + ", ".join(f"{st} {w[st]['rel_dcd']:.2e}"
+                                                          for st in ("f32", "f16", bench.FLOOR)))
+        assert w["finite_f64"] and w["cd_f64"] > 0
+        assert w["f32"]["met"] and w["f16"]["met"]
+        assert w[bench.FLOOR]["rel_dcd"] < w["f32"]["rel_dcd"]
+    # measured on MI355X: FP32 1.9e-6 / 2.0e-6, FP16 5.9e-4 / 5.2e-4, floor 8e-10 / 4e-9 (the runs
+    # are deterministic: the same seeds give the same values)
+    d = bench.seed_distribution(windows, ["f32", "f16", bench.FLOOR])
+    assert d["f32"]["runs"] == 2 and d["f32"]["met"] == 2 and d["f16"]["met"] == 2
