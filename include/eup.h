@@ -64,9 +64,10 @@ extern "C" {
 /* flags */
 #define EUP_FLAG_NO_TILE_SKIP 1u   /* physics mode: process every tile (disables the all-solid
                                       tile skip and the pure-fluid fast path) */
-#define EUP_FLAG_SHIFTED_DEVIATION 2u /* FP16/BF16 storage holds f_q - f_eq_q(1, (u_inflow, 0, 0))
-                                         instead of f_q - w_q: smaller stored values in the free
-                                         stream, smaller rounding error */
+#define EUP_FLAG_SHIFTED_DEVIATION 2u /* the 16-bit storages (FP16, BF16, e3m12) hold
+                                         f_q - f_eq_q(1, (u_inflow, 0, 0)) instead of f_q - w_q
+                                         (e3m12: divided by w_q): smaller stored values in the
+                                         free stream, smaller rounding error */
 
 typedef struct eup_handle eup_handle;
 
